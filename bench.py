@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark: CD10 periodic first derivative at 512^3 on one MI355X.
+
+Workload (precision-study configuration): a 512^3 FP64 isotropic random field
+with a k^-5/3 spectrum (|k| <= 170, seeded), CD10 periodic ddx.  One "step"
+is one application of the operator to the whole field (one kernel launch);
+inputs are resident in HBM before the timed region.
+
+Headline `value`: Gpts/s = 512^3 / t_step / 1e9 of the FP64 banded ddx.
+Also reported: ddy / ddz, the 3-axis aggregate, the FP32 banded solve and the
+MFMA dense forms, each with max and RMS relative error against the FP64
+closed form (the periodic operator applied exactly through the FFT), the
+roofline of the dominant kernel, and the CPU baseline (the C restatement of
+the reference's banded algorithm, oracle/pade_banded.c, on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
+FP32_MFMA_TFLOPS = 157.3
+BF16_MFMA_TFLOPS = 2500.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(steps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / steps
+
+
+def errors(out, ref):
+    d = (out.double() - ref)
+    mx = float(d.abs().max() / ref.abs().max())
+    rms = float(torch_sqrt_mean(d * d) / torch_sqrt_mean(ref * ref))
+    return mx, rms
+
+
+def torch_sqrt_mean(x):
+    return float(x.mean().sqrt())
+
+
+def cpu_baseline(n, dx, nthreads):
+    """CD10 periodic ddx on a bounded sample of the same workload, host cores."""
+    import numpy as np
+    from oracle import banded
+
+    banded.build()
+    rng = np.random.default_rng(5)
+    na, nb = n, 16                       # 512 x 512 x 16 pencils slab = 4.2M points
+    f = np.asfortranarray(rng.standard_normal((n, na, nb)))
+    out = np.empty_like(f, order="F")
+    op = banded.BandedOp("cd10", "d1", n, dx, True)
+    res = {}
+    for th in sorted({1, nthreads}):
+        op.apply(f, 0, nthreads=th, out=out)
+        reps = []
+        t_end = time.perf_counter() + (6.0 if th == 1 else 4.0)
+        while len(reps) < 3 or (time.perf_counter() < t_end and len(reps) < 15):
+            t0 = time.perf_counter()
+            op.apply(f, 0, nthreads=th, out=out)
+            reps.append(time.perf_counter() - t0)
+        reps.sort()
+        res[th] = f.size / reps[len(reps) // 2] / 1e9
+    return res, f"{n}x{na}x{nb} slab of pencils (x-derivative), median of repeated applies"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="time only the headline kernel")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build targets one GPU: the operator path is measured on one "
+                                   "MI355X; multi-GPU sharding is design-only (run with --gpus 1)"}))
+        sys.exit(2)
+
+    import torch
+
+    import padeops_amd as po
+    from padeops_amd import fields, spectral, _lib
+
+    n = args.n
+    dx = 2 * math.pi / n
+    dev = "cuda:0"
+    log(f"bench: generating {n}^3 k^-5/3 field on {dev}")
+    f = fields.kolmogorov((n, n, n), kmax=min(170, n // 3), seed=3, device=dev)
+    df = po.fortran_empty((n, n, n), device=dev)
+    op = po.cd10()
+    assert op.init(n, dx, True) == 0
+
+    log("bench: timing FP64 ddx")
+    ms = timed(lambda: op.dd1(f, df, n, n), args.steps, args.warmup)
+    pts = n ** 3
+    value = pts / (ms * 1e-3) / 1e9
+    achieved = 16.0 * pts / (ms * 1e-3) / 1e9
+
+    extra = {}
+    if not args.no_extra:
+        # exact periodic operator through the FFT (FP64): the error reference
+        fc = f.permute(2, 1, 0)                       # C-contiguous view (n3, n2, n1)
+        ref_x = spectral.apply_fourier("cd10", "d1", fc, 2, dx)
+        op.dd1(f, df, n, n)
+        torch.cuda.synchronize()
+        extra["fp64_x_err"] = errors(df.permute(2, 1, 0), ref_x)
+        log("bench: timing FP64 ddy, ddz")
+        ms_y = timed(lambda: op.dd2(f, df, n, n), args.steps, args.warmup)
+        ms_z = timed(lambda: op.dd3(f, df, n, n), args.steps, args.warmup)
+        extra["fp64_gpts"] = {"x": value, "y": pts / (ms_y * 1e-3) / 1e9, "z": pts / (ms_z * 1e-3) / 1e9,
+                              "xyz_aggregate": pts / ((ms + ms_y + ms_z) * 1e-3) / 1e9}
+        log("bench: FP32 banded")
+        f32 = po.fortran_empty((n, n, n), dtype=torch.float32, device=dev)
+        f32.copy_(f)
+        d32 = po.fortran_empty((n, n, n), dtype=torch.float32, device=dev)
+        ms32 = timed(lambda: op.dd1(f32, d32, n, n), args.steps, args.warmup)
+        op.dd1(f32, d32, n, n)
+        torch.cuda.synchronize()
+        extra["fp32_banded"] = {"gpts": pts / (ms32 * 1e-3) / 1e9,
+                                "hbm_frac": 8.0 * pts / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                "err_max_rms": errors(d32.permute(2, 1, 0), ref_x)}
+        del f32, d32
+        for name, prec in (("dense_fp32", _lib.PADE_DENSE_F32), ("dense_bf16x3", _lib.PADE_DENSE_BF16X3)):
+            try:
+                d32 = po.fortran_empty((n, n, n), dtype=torch.float32, device=dev)
+                run = lambda: op.dense_apply(_lib.PADE_D1, 1, f, d32, n, n, prec)  # noqa: E731
+                run()
+                torch.cuda.synchronize()
+                msd = timed(run, max(5, args.steps // 5), 2)
+                extra[name] = {"gpts": pts / (msd * 1e-3) / 1e9,
+                               "err_max_rms": errors(d32.permute(2, 1, 0), ref_x)}
+                del d32
+            except po.PadeError as e:
+                extra[name] = {"unavailable": str(e)}
+        del ref_x
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: CPU baseline")
+        nthreads = max(1, min(16, os.cpu_count() or 1))
+        res, sample = cpu_baseline(n, dx, nthreads)
+        cpu = {"value": res[nthreads], "unit": "Gpts/s", "cores": nthreads, "kind": "port",
+               "sample": sample, "single_thread_gpts": res[1]}
+
+    line = {
+        "metric": "Gpts/s per CD10 3D derivative at 512^3: FP64 banded vs MFMA form at <=1e-6 err",
+        "value": value,
+        "unit": "Gpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded k^-5/3 isotropic field, |k|<=170, zero mean, unit RMS)",
+        "config": {"workload": f"CD10 periodic ddx, {n}^3 FP64 (precision-study configuration)",
+                   "grid": [n, n, n], "operator": "cd10 d1 periodic", "axis": "x"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,122 @@
+/*
+ * padeops_hip.h -- C ABI of the MI355X compact-difference operator library
+ * (libpadeops_hip.so).
+ *
+ * This is the drop-in boundary for PadeOps' compact operators: the
+ * cd06 / cd10 first and second derivatives and the cf90 compact filter,
+ * applied along x, y or z of a Fortran-ordered 3-D array (i fastest).
+ * Plain pointers, sizes and integer codes only; a Fortran iso_c_binding shim
+ * (padeops_amd/fortran/) re-exports the reference's module surface on top.
+ *
+ * Conventions
+ *   axis 1: f(n, na, nb)   derivative along the contiguous axis (dd1 / d2d1 / filter1)
+ *   axis 2: f(na, n, nb)   (dd2 / d2d2 / filter2)
+ *   axis 3: f(na, nb, n)   (dd3 / d2d3 / filter3)
+ *   bc codes: 0 one-sided closure, 1 symmetric, -1 antisymmetric; an absent
+ *   optional bc in the reference means 0, so callers pass 0.
+ *   Device-pointer entry points are asynchronous on `stream` (a hipStream_t,
+ *   NULL = default stream); f and df must not overlap.
+ *
+ * Return codes
+ *   0      success
+ *   2,3,7  init size errors, as the reference's init returns them
+ *   21     operation not provided for this operator kind
+ *   324    bc code outside {-1,0,1} (the reference's GracefulExit code)
+ *   400    invalid argument (null pointer, size mismatch, bad axis)
+ *   1000+e HIP runtime error e
+ * pade_last_error() gives a message for the last non-zero code of the
+ * calling thread.
+ */
+#ifndef PADEOPS_HIP_H
+#define PADEOPS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pade_kind { PADE_CD06 = 1, PADE_CD10 = 2, PADE_CF90 = 3 };
+enum pade_oper { PADE_D1 = 1, PADE_D2 = 2, PADE_FILTER = 3 };
+enum pade_dense_prec { PADE_DENSE_F32 = 1, PADE_DENSE_BF16X3 = 2 };
+enum pade_dtype { PADE_F64 = 0, PADE_F32 = 1 };
+
+typedef struct pade_op_s *pade_op_t;
+
+/* Create an operator for pencils of n points.
+ * Replaces cd06%init (src/derivatives/cd06.F90:129), cd10%init
+ * (src/derivatives/cd10.F90:195) and, with dx ignored, cf90%init
+ * (src/filters/cf90.F90:107).  bc1/bcn matter only for bounded cd06 (they
+ * select its LHS closure at init, cd06.F90:264-327); cd10 / cf90 take the bc
+ * at apply time.  Returns the reference's ierr (2, 3, 7) for too small n. */
+int pade_op_create(int kind, int n, double dx, int periodic, int bc1, int bcn, pade_op_t *op);
+
+/* Release device tables.  Replaces cd06/cd10/cf90 %destroy
+ * (cd06.F90:203-219, cd10.F90:317-349, cf90.F90:177-196). */
+int pade_op_destroy(pade_op_t op);
+
+/* Pencil length.  Replaces %GetSize (cd06.F90:123-127, cd10.F90:189-193). */
+int pade_op_size(pade_op_t op);
+
+/* FP64 apply on device arrays.  oper PADE_D1 replaces dd1/dd2/dd3
+ * (cd10.F90:2029,2099,2169; cd06.F90:775,797,819), PADE_D2 replaces
+ * d2d1/d2d2/d2d3 (cd10.F90:2239,2309,2379; cd06 has no reference version,
+ * see the operator definition in the library documentation), PADE_FILTER
+ * replaces filter1/filter2/filter3 (cf90.F90:1020,1090,1160). */
+int pade_op_apply(pade_op_t op, int oper, int axis, const double *f, double *df, int na, int nb,
+                  int bc1, int bcn, void *stream);
+
+/* FP32 banded apply (same operator, FP32 storage and arithmetic; factors
+ * computed in FP64 and rounded).  No reference counterpart (precision study). */
+int pade_op_apply_f32(pade_op_t op, int oper, int axis, const float *f, float *df, int na, int nb,
+                      int bc1, int bcn, void *stream);
+
+/* Adjoint apply: df = B^T A^{-T} f, including the transposed boundary
+ * closures.  No reference counterpart (backward pass of the autograd entry
+ * point). */
+int pade_op_apply_adjoint(pade_op_t op, int oper, int axis, const double *f, double *df, int na,
+                          int nb, int bc1, int bcn, void *stream);
+int pade_op_apply_adjoint_f32(pade_op_t op, int oper, int axis, const float *f, float *df, int na,
+                              int nb, int bc1, int bcn, void *stream);
+
+/* Host-array apply for drop-in callers whose data lives in host memory:
+ * copies in, applies on the device, copies out, synchronous.  Same meaning as
+ * pade_op_apply (the reference's dd*, d2d*, filter* take host arrays). */
+int pade_op_apply_host(pade_op_t op, int oper, int axis, const double *f, double *df, int na,
+                       int nb, int bc1, int bcn);
+
+/* Dense-operator form: df = D f with D = A^{-1} B precomputed in FP64 and
+ * applied as an MFMA GEMM over all pencils.  prec PADE_DENSE_F32 (FP32
+ * operands, FP32 accumulate) or PADE_DENSE_BF16X3 (3-term split BF16 of
+ * both operands, FP32 accumulate).  in/out dtype PADE_F64 or PADE_F32.
+ * No reference counterpart (precision study). */
+int pade_op_dense_apply(pade_op_t op, int oper, int axis, const void *f, int f_dtype, void *df,
+                        int df_dtype, int na, int nb, int bc1, int bcn, int prec, void *stream);
+
+/* Pencil re-orientation on one GPU.  2DECOMP's transpose_x_to_y / y_to_z /
+ * z_to_y / y_to_x (2decomp_fft/src/transpose_x_to_y.f90:14-91 and siblings)
+ * keep (i,j,k) order and only move data between ranks, so on one rank they
+ * are copies: pade_pencil_copy.  pade_permute3 is the axis-permuting
+ * re-orientation dst(perm(i,j,k)) = src(i,j,k) for elem_size 4 or 8;
+ * perm 1: (i,j,k)->(j,i,k), 2: (i,j,k)->(k,j,i), 3: (i,j,k)->(i,k,j). */
+int pade_pencil_copy(const void *src, void *dst, long long nbytes, void *stream);
+int pade_permute3(const void *src, void *dst, int elem_size, int n1, int n2, int n3, int perm,
+                  void *stream);
+
+/* Host-only planning export (no GPU needed): writes the FP64 table the
+ * kernels consume for (kind, oper, n, dx, periodic, bc1, bcn, adjoint) with
+ * partition size M (M = 0: library choice), and meta[12] =
+ * {n, M, P, periodic, nbl, nbr, off_bl, off_br, off_lu, off_S, off_Q, len}.
+ * Pass tab = NULL to query len.  Used by tests to check the planner on CPU. */
+int pade_plan_export(int kind, int oper, int n, double dx, int periodic, int bc1, int bcn,
+                     int adjoint, int M, double *tab, long long tab_len, int *meta);
+
+/* Message for the last non-zero return code of this thread. */
+const char *pade_last_error(void);
+
+/* Library version (major*10000 + minor*100 + patch). */
+int pade_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PADEOPS_HIP_H */

@@ -1,0 +1,112 @@
+"""In-tree build of the native pieces of padeops_amd.
+
+* ``padeops_amd/lib/libpadeops_hip.so`` -- HIP kernels + C ABI, gfx950.
+* ``padeops_amd/lib/libpadeops_f.so``   -- Fortran iso_c_binding modules
+  (built when a Fortran compiler is present).
+
+Run ``python -m padeops_amd._build`` or call :func:`build`.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIBDIR, "libpadeops_hip.so")
+FLIB = os.path.join(LIBDIR, "libpadeops_f.so")
+FSRC = os.path.join(HERE, "fortran")
+
+HIP_SOURCES = ["pade_plan.cpp", "pade_kernels.hip", "pade_dense.hip", "pade_capi.cpp"]
+FORTRAN_SOURCES = ["padeops_c.F90", "cd06stuff.F90", "cd10stuff.F90", "cf90stuff.F90",
+                   "derivatives_mod.F90", "filters_mod.F90"]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # keep packed-FP32 instructions out of the generated code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+]
+
+
+def _hipcc():
+    for c in ("/opt/rocm/bin/hipcc", shutil.which("hipcc") or ""):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _fc():
+    for c in ("/opt/rocm/bin/amdflang", "/opt/rocm/lib/llvm/bin/flang", shutil.which("gfortran") or ""):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+def build_hip(verbose=False):
+    os.makedirs(LIBDIR, exist_ok=True)
+    objs = []
+    hipcc = _hipcc()
+    objdir = os.path.join(ROOT, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    procs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc, "-x", "hip", *HIP_CC(), "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+        objs.append(obj)
+    failed = []
+    for src, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            failed.append(src)
+            sys.stderr.write(out.decode(errors="replace"))
+        elif verbose and out:
+            sys.stderr.write(out.decode(errors="replace"))
+    if failed:
+        raise RuntimeError("HIP compilation failed: " + ", ".join(failed))
+    tmp = LIB + ".tmp"
+    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def HIP_CC():
+    return list(HIPCC_FLAGS) + ["-I", os.path.join(ROOT, "include")]
+
+
+def build_fortran(verbose=False):
+    fc = _fc()
+    if fc is None:
+        return None
+    srcs = [os.path.join(FSRC, s) for s in FORTRAN_SOURCES if os.path.exists(os.path.join(FSRC, s))]
+    if len(srcs) != len(FORTRAN_SOURCES):
+        return None
+    moddir = os.path.join(ROOT, "_build", "fmod")
+    os.makedirs(moddir, exist_ok=True)
+    tmp = FLIB + ".tmp"
+    cmd = [fc, "-O2", "-fPIC", "-shared", "-J", moddir, "-o", tmp, *srcs,
+           "-L", LIBDIR, "-lpadeops_hip", "-Wl,-rpath,$ORIGIN", "-lc", "-lpthread", "-lm"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=moddir)
+    os.replace(tmp, FLIB)
+    return FLIB
+
+
+def build(verbose=False):
+    lib = build_hip(verbose)
+    build_fortran(verbose)
+    return lib
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(LIB)

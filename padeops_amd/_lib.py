@@ -1,0 +1,78 @@
+"""Loader for ``libpadeops_hip.so`` (ctypes over the C ABI in include/padeops_hip.h).
+
+The product path always runs through this library; there is no CPU fallback.
+If the library is missing the import of an operator fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libpadeops_hip.so")
+
+PADE_CD06, PADE_CD10, PADE_CF90 = 1, 2, 3
+PADE_D1, PADE_D2, PADE_FILTER = 1, 2, 3
+PADE_DENSE_F32, PADE_DENSE_BF16X3 = 1, 2
+PADE_F64, PADE_F32 = 0, 1
+
+# every symbol include/padeops_hip.h declares
+EXPORTS = [
+    "pade_op_create", "pade_op_destroy", "pade_op_size", "pade_op_apply", "pade_op_apply_f32",
+    "pade_op_apply_adjoint", "pade_op_apply_adjoint_f32", "pade_op_apply_host",
+    "pade_op_dense_apply", "pade_pencil_copy", "pade_permute3", "pade_plan_export",
+    "pade_last_error", "pade_version",
+]
+
+_lib = None
+
+
+class PadeError(RuntimeError):
+    """Non-zero return code of the native library."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"padeops_hip error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m padeops_amd._build` (or __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    c_int, c_double, c_void_p, c_ll = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_longlong
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.pade_op_create.argtypes = [c_int, c_int, c_double, c_int, c_int, c_int, ctypes.POINTER(c_void_p)]
+    lib.pade_op_destroy.argtypes = [c_void_p]
+    lib.pade_op_size.argtypes = [c_void_p]
+    apply_sig = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p]
+    for name in ("pade_op_apply", "pade_op_apply_f32", "pade_op_apply_adjoint", "pade_op_apply_adjoint_f32"):
+        getattr(lib, name).argtypes = apply_sig
+    lib.pade_op_apply_host.argtypes = [c_void_p, c_int, c_int, dp, dp, c_int, c_int, c_int, c_int]
+    lib.pade_op_dense_apply.argtypes = [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_int,
+                                        c_int, c_int, c_int, c_int, c_void_p]
+    lib.pade_pencil_copy.argtypes = [c_void_p, c_void_p, c_ll, c_void_p]
+    lib.pade_permute3.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p]
+    lib.pade_plan_export.argtypes = [c_int, c_int, c_int, c_double, c_int, c_int, c_int, c_int, c_int, dp,
+                                     c_ll, ip]
+    lib.pade_last_error.restype = ctypes.c_char_p
+    lib.pade_last_error.argtypes = []
+    lib.pade_version.argtypes = []
+    for name in EXPORTS:
+        fn = getattr(lib, name)
+        if fn.restype is ctypes.c_int or name not in ("pade_last_error",):
+            if name != "pade_last_error":
+                fn.restype = c_int
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().pade_last_error().decode(errors="replace")
+        raise PadeError(rc, msg)
+    return rc

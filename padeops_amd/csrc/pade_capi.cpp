@@ -1,0 +1,482 @@
+// C ABI of libpadeops_hip: operator handles, plan tables, dispatch, errors.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/padeops_hip.h"
+#include "pade_dense.h"
+#include "pade_kernels.h"
+#include "pade_plan.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *where)
+{
+    return fail(1000 + int(e), std::string(where) + ": " + hipGetErrorString(e));
+}
+
+// One factor variant (partition size M) of one plan, as laid out in the
+// device table.
+struct Variant {
+    int M = 0, P = 0;
+    long long off = 0;  // element offset of this variant's table in the plan buffer
+    int nbl = 0, nbr = 0;
+    int off_bl = 0, off_br = 0, off_lu = 0, off_S = 0, off_Q = 0;
+    int len = 0;
+};
+
+struct PlanKey {
+    int oper, bc1, bcn, adj;
+    bool operator<(const PlanKey &o) const
+    {
+        return std::tie(oper, bc1, bcn, adj) < std::tie(o.oper, o.bc1, o.bcn, o.adj);
+    }
+};
+
+struct Plan {
+    std::vector<Variant> spike;  // register-path variants (M in {16,32,64})
+    Variant generic;             // whole pencil as one partition
+    bool has_generic = false;
+    pade::OpMatrices mats;       // for the dense form
+};
+
+// Assemble the table of one variant; returns 0 or an error code.
+int build_variant(const pade::OpMatrices &m, int M, std::vector<double> &tab, Variant &v)
+{
+    pade::SpikeFactors sf;
+    if (int rc = pade::spike_factor(m, M, sf)) return rc;
+    v.M = M;
+    v.P = sf.P;
+    v.nbl = m.nbl;
+    v.nbr = m.nbr;
+    v.off_bl = 16;
+    v.off_br = v.off_bl + pade::kBRowsDev * pade::kBColsDev;
+    v.off_lu = v.off_br + pade::kBRowsDev * pade::kBColsDev;
+    v.off_S = v.off_lu + int(sf.lu.size());
+    v.off_Q = v.off_S + int(sf.S.size());
+    v.len = v.off_Q + int(sf.Q.size());
+    v.len = (v.len + 31) & ~31;
+    v.off = (long long)tab.size();
+    tab.resize(tab.size() + v.len, 0.0);
+    double *t = &tab[v.off];
+    for (int k = 0; k < 9; k++) t[k] = m.stencil[k];
+    for (int r = 0; r < pade::kBRowsDev; r++)
+        for (int c = 0; c < pade::kBColsDev; c++) {
+            t[v.off_bl + r * pade::kBColsDev + c] = m.bl[r][c];
+            t[v.off_br + r * pade::kBColsDev + c] = m.br[r][c];
+        }
+    std::copy(sf.lu.begin(), sf.lu.end(), t + v.off_lu);
+    std::copy(sf.S.begin(), sf.S.end(), t + v.off_S);
+    std::copy(sf.Q.begin(), sf.Q.end(), t + v.off_Q);
+    return 0;
+}
+
+int operator_matrices(int kind, int oper, int n, double dx, bool periodic, int bc1, int bcn, bool adj,
+                      pade::OpMatrices &out)
+{
+    pade::OpMatrices fwd;
+    if (int rc = pade::build_operator(kind, oper, n, dx, periodic, bc1, bcn, fwd)) return rc;
+    if (adj) pade::transpose_operator(fwd, out);
+    else out = fwd;
+    return 0;
+}
+
+bool oper_valid(int kind, int oper)
+{
+    if (kind == pade::CF90) return oper == pade::FILTER;
+    return oper == pade::D1 || oper == pade::D2;
+}
+
+}  // namespace
+
+struct pade_op_s {
+    int kind = 0, n = 0;
+    double dx = 1.0;
+    bool periodic = true;
+    int init_bc1 = 0, init_bcn = 0;
+    int device = 0;
+    std::map<PlanKey, Plan> plans;
+    double *dtab64 = nullptr;
+    float *dtab32 = nullptr;
+    long long tab_len = 0;
+    std::mutex mu;
+    pade::DenseCache dense;
+};
+
+namespace {
+
+int plan_key_bc(const pade_op_s *op, int bc) { return op->kind == pade::CD06 ? 0 : bc; }
+
+// All (oper, bc1, bcn, adj) plans an operator can be asked for.
+int build_all_plans(pade_op_s *op)
+{
+    std::vector<double> tab;
+    std::vector<int> opers;
+    if (op->kind == pade::CF90) opers = {pade::FILTER};
+    else opers = {pade::D1, pade::D2};
+    std::vector<std::pair<int, int>> bcs;
+    if (op->periodic || op->kind == pade::CD06) bcs = {{0, 0}};
+    else
+        for (int a = -1; a <= 1; a++)
+            for (int b = -1; b <= 1; b++) bcs.push_back({a, b});
+    for (int oper : opers) {
+        if (op->kind == pade::CD06 && oper == pade::D2 && !op->periodic) continue;  // composed
+        for (auto [b1, bn] : bcs)
+            for (int adj = 0; adj < 2; adj++) {
+                if (adj && op->n < 16) continue;  // adjoint needs n >= 16 (boundary rows of B^T)
+                const int cb1 = op->kind == pade::CD06 ? op->init_bc1 : b1;
+                const int cbn = op->kind == pade::CD06 ? op->init_bcn : bn;
+                Plan pl;
+                if (int rc = operator_matrices(op->kind, oper, op->n, op->dx, op->periodic, cb1, cbn,
+                                               adj != 0, pl.mats))
+                    return fail(rc, "operator construction failed");
+                for (int M : {16, 32, 64}) {
+                    if (op->n % M || op->n / M > 8) continue;
+                    Variant v;
+                    if (build_variant(pl.mats, M, tab, v) == 0) pl.spike.push_back(v);
+                }
+                if (op->n >= 4 && build_variant(pl.mats, op->n, tab, pl.generic) == 0) pl.has_generic = true;
+                op->plans[{oper, b1, bn, adj}] = std::move(pl);
+            }
+    }
+    op->tab_len = (long long)tab.size();
+    hipError_t e = hipMalloc(&op->dtab64, tab.size() * sizeof(double));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(tables)");
+    e = hipMalloc(&op->dtab32, tab.size() * sizeof(float));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(tables)");
+    std::vector<float> t32(tab.begin(), tab.end());
+    e = hipMemcpy(op->dtab64, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(tables)");
+    e = hipMemcpy(op->dtab32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(tables)");
+    return 0;
+}
+
+template <typename T>
+__global__ void fill_kernel(T *dst, long long count, T v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = v;
+}
+
+template <typename T>
+int fill_zero(T *dst, long long count, hipStream_t s)
+{
+    const int bs = 256;
+    const long long nb = (count + bs - 1) / bs;
+    hipLaunchKernelGGL((fill_kernel<T>), dim3((unsigned)nb), dim3(bs), 0, s, dst, count, T(0));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : hip_fail(e, "fill");
+}
+
+// Choose the variant for this launch: the largest M whose workgroup count
+// keeps the chip busy; the smallest otherwise.
+const Variant *pick_variant(const Plan &pl, long long npencils)
+{
+    if (pl.spike.empty()) return nullptr;
+    const long long groups = (npencils + 63) / 64;
+    const Variant *best = &pl.spike.front();  // smallest M first
+    for (const Variant &v : pl.spike)
+        if (groups * v.P >= 2048 || v.M == pl.spike.front().M) best = &v;
+    // prefer the largest M that still gives >= 2048 waves
+    const Variant *pick = nullptr;
+    for (const Variant &v : pl.spike)
+        if (groups * v.P >= 2048) pick = &v;
+    return pick ? pick : best;
+}
+
+template <typename T>
+int apply_impl(pade_op_t op, int oper, int axis, const T *f, T *df, int na, int nb, int bc1, int bcn,
+               bool adj, hipStream_t s);
+
+template <typename T>
+int apply_one(pade_op_t op, const Plan &pl, int axis, const T *f, T *df, int na, int nb, bool adj,
+              hipStream_t s)
+{
+    const int n = op->n;
+    pade::ApplyArgs<T> a{};
+    a.f = f;
+    a.out = df;
+    a.n = n;
+    a.periodic = op->periodic ? 1 : 0;
+    a.npencils = (long long)na * nb;
+    int mode = 0;
+    if (axis == 1) {
+        a.A1 = a.npencils; a.B1 = n; a.B2 = 0; a.stride = 1; mode = 1;
+    } else if (axis == 2) {
+        a.A1 = na; a.B1 = 1; a.B2 = (long long)na * n; a.stride = na;
+    } else {
+        a.A1 = a.npencils; a.B1 = 1; a.B2 = 0; a.stride = (long long)na * nb;
+    }
+    const T *base = std::is_same<T, double>::value ? (const T *)op->dtab64 : (const T *)op->dtab32;
+    const Variant *v = pick_variant(pl, a.npencils);
+    const bool use_generic = (v == nullptr);
+    if (use_generic) {
+        if (!pl.has_generic) return fail(21, "no solver variant for this size");
+        v = &pl.generic;
+    }
+    a.tab = base + v->off;
+    a.M = v->M;
+    a.P = v->P;
+    a.nbl = v->nbl;
+    a.nbr = v->nbr;
+    a.off_bl = v->off_bl;
+    a.off_br = v->off_br;
+    a.off_lu = v->off_lu;
+    a.off_S = v->off_S;
+    a.off_Q = v->off_Q;
+    hipError_t e;
+    if (!use_generic) {
+        e = pade::launch_spike<T>(a, v->M, adj, mode, s);
+        if (e != hipSuccess) return hip_fail(e, "spike_apply launch");
+        return 0;
+    }
+    T *work = nullptr;
+    if (adj) {
+        e = hipMallocAsync((void **)&work, sizeof(T) * (size_t)a.npencils * n, s);
+        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(work)");
+    }
+    e = pade::launch_generic<T>(a, adj, work, s);
+    if (work) hipFreeAsync(work, s);
+    if (e != hipSuccess) return hip_fail(e, "generic_apply launch");
+    return 0;
+}
+
+template <typename T>
+int apply_impl(pade_op_t op, int oper, int axis, const T *f, T *df, int na, int nb, int bc1, int bcn,
+               bool adj, hipStream_t s)
+{
+    if (!op) return fail(400, "null operator");
+    if (!f || !df) return fail(400, "null array");
+    if (axis < 1 || axis > 3) return fail(400, "axis must be 1, 2 or 3");
+    if (na < 1 || nb < 1) return fail(400, "na and nb must be positive");
+    if (!oper_valid(op->kind, oper)) return fail(21, "operation not provided for this operator kind");
+    if (op->kind != pade::CD06) {
+        auto ok = [](int b) { return b == 0 || b == 1 || b == -1; };
+        if (!ok(bc1) || !ok(bcn))
+            return fail(324, "Incorrect boundary specification (should be 0, 1 or -1)");
+    }
+    const long long count = (long long)na * nb * op->n;
+    if (op->n == 1) {
+        if (oper == pade::FILTER) {
+            hipError_t e = hipMemcpyAsync(df, f, sizeof(T) * count, hipMemcpyDeviceToDevice, s);
+            return e == hipSuccess ? 0 : hip_fail(e, "copy");
+        }
+        return fill_zero(df, count, s);
+    }
+    if (op->kind == pade::CD06 && oper == pade::D2 && !op->periodic) {
+        // bounded cd06 second derivative: the first derivative applied twice
+        T *tmp = nullptr;
+        hipError_t e = hipMallocAsync((void **)&tmp, sizeof(T) * count, s);
+        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(tmp)");
+        int rc = apply_impl<T>(op, pade::D1, axis, f, tmp, na, nb, bc1, bcn, adj, s);
+        if (!rc) rc = apply_impl<T>(op, pade::D1, axis, tmp, df, na, nb, bc1, bcn, adj, s);
+        hipFreeAsync(tmp, s);
+        return rc;
+    }
+    const int k1 = op->periodic ? 0 : plan_key_bc(op, bc1);
+    const int kn = op->periodic ? 0 : plan_key_bc(op, bcn);
+    auto it = op->plans.find({oper, k1, kn, adj ? 1 : 0});
+    if (it == op->plans.end())
+        return fail(21, adj ? "adjoint needs n >= 16" : "no plan for this operation");
+    return apply_one<T>(op, it->second, axis, f, df, na, nb, adj, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pade_version(void) { return 100; }
+
+const char *pade_last_error(void) { return g_last_error.c_str(); }
+
+int pade_op_create(int kind, int n, double dx, int periodic, int bc1, int bcn, pade_op_t *out)
+{
+    if (!out) return fail(400, "null output handle");
+    *out = nullptr;
+    if (kind < pade::CD06 || kind > pade::CF90) return fail(400, "unknown operator kind");
+    if (n < 1) return fail(400, "n must be positive");
+    if (kind != pade::CF90 && !(dx > 0.0) && n > 1) return fail(400, "dx must be positive");
+    if (int ierr = pade::init_error(kind, n, periodic != 0))
+        return fail(ierr, "grid too small for this operator");
+    auto op = std::make_unique<pade_op_s>();
+    op->kind = kind;
+    op->n = n;
+    op->dx = kind == pade::CF90 ? 1.0 : dx;
+    op->periodic = periodic != 0;
+    op->init_bc1 = bc1;
+    op->init_bcn = bcn;
+    hipError_t e = hipGetDevice(&op->device);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (n > 1) {
+        if (int rc = build_all_plans(op.get())) {
+            if (op->dtab64) hipFree(op->dtab64);
+            if (op->dtab32) hipFree(op->dtab32);
+            return rc;
+        }
+    }
+    *out = op.release();
+    return 0;
+}
+
+int pade_op_destroy(pade_op_t op)
+{
+    if (!op) return 0;
+    if (op->dtab64) hipFree(op->dtab64);
+    if (op->dtab32) hipFree(op->dtab32);
+    pade::dense_release(op->dense);
+    delete op;
+    return 0;
+}
+
+int pade_op_size(pade_op_t op) { return op ? op->n : -1; }
+
+int pade_op_apply(pade_op_t op, int oper, int axis, const double *f, double *df, int na, int nb,
+                  int bc1, int bcn, void *stream)
+{
+    return apply_impl<double>(op, oper, axis, f, df, na, nb, bc1, bcn, false, (hipStream_t)stream);
+}
+
+int pade_op_apply_f32(pade_op_t op, int oper, int axis, const float *f, float *df, int na, int nb,
+                      int bc1, int bcn, void *stream)
+{
+    return apply_impl<float>(op, oper, axis, f, df, na, nb, bc1, bcn, false, (hipStream_t)stream);
+}
+
+int pade_op_apply_adjoint(pade_op_t op, int oper, int axis, const double *f, double *df, int na,
+                          int nb, int bc1, int bcn, void *stream)
+{
+    return apply_impl<double>(op, oper, axis, f, df, na, nb, bc1, bcn, true, (hipStream_t)stream);
+}
+
+int pade_op_apply_adjoint_f32(pade_op_t op, int oper, int axis, const float *f, float *df, int na,
+                              int nb, int bc1, int bcn, void *stream)
+{
+    return apply_impl<float>(op, oper, axis, f, df, na, nb, bc1, bcn, true, (hipStream_t)stream);
+}
+
+int pade_op_apply_host(pade_op_t op, int oper, int axis, const double *f, double *df, int na, int nb,
+                       int bc1, int bcn)
+{
+    if (!op) return fail(400, "null operator");
+    if (!f || !df) return fail(400, "null array");
+    if (na < 1 || nb < 1) return fail(400, "na and nb must be positive");
+    const size_t bytes = sizeof(double) * (size_t)na * nb * op->n;
+    double *df_d = nullptr, *f_d = nullptr;
+    hipError_t e = hipMalloc(&f_d, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    e = hipMalloc(&df_d, bytes);
+    if (e != hipSuccess) { hipFree(f_d); return hip_fail(e, "hipMalloc"); }
+    int rc = 0;
+    e = hipMemcpy(f_d, f, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy H2D");
+    if (!rc) rc = apply_impl<double>(op, oper, axis, f_d, df_d, na, nb, bc1, bcn, false, nullptr);
+    if (!rc) {
+        e = hipMemcpy(df, df_d, bytes, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy D2H");
+    }
+    hipFree(f_d);
+    hipFree(df_d);
+    return rc;
+}
+
+int pade_op_dense_apply(pade_op_t op, int oper, int axis, const void *f, int f_dtype, void *df,
+                        int df_dtype, int na, int nb, int bc1, int bcn, int prec, void *stream)
+{
+    if (!op) return fail(400, "null operator");
+    if (!f || !df) return fail(400, "null array");
+    if (axis < 1 || axis > 3) return fail(400, "axis must be 1, 2 or 3");
+    if (na < 1 || nb < 1) return fail(400, "na and nb must be positive");
+    if (!oper_valid(op->kind, oper)) return fail(21, "operation not provided for this operator kind");
+    if (op->kind != pade::CD06) {
+        auto ok = [](int b) { return b == 0 || b == 1 || b == -1; };
+        if (!ok(bc1) || !ok(bcn))
+            return fail(324, "Incorrect boundary specification (should be 0, 1 or -1)");
+    }
+    if (prec != PADE_DENSE_F32 && prec != PADE_DENSE_BF16X3) return fail(400, "unknown dense precision");
+    if ((f_dtype != PADE_F64 && f_dtype != PADE_F32) || (df_dtype != PADE_F64 && df_dtype != PADE_F32))
+        return fail(400, "unknown dtype");
+    if (op->n < 16 || op->n % 16 != 0) return fail(21, "dense form needs n a multiple of 16");
+    if (op->kind == pade::CD06 && oper == pade::D2 && !op->periodic)
+        return fail(21, "dense form of the composed bounded cd06 d2 is not provided");
+    const int k1 = op->periodic ? 0 : plan_key_bc(op, bc1);
+    const int kn = op->periodic ? 0 : plan_key_bc(op, bcn);
+    auto it = op->plans.find({oper, k1, kn, 0});
+    if (it == op->plans.end()) return fail(21, "no plan for this operation");
+    const pade::DenseMatrix *D = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(op->mu);
+        int rc = pade::dense_get(op->dense, it->second.mats, oper * 100 + (k1 + 1) * 10 + (kn + 1), &D);
+        if (rc) return fail(rc >= 1000 ? rc : 21, "dense operator construction failed");
+    }
+    hipError_t e = pade::dense_launch(*D, axis, f, f_dtype, df, df_dtype, na, nb, prec, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "dense launch");
+    return 0;
+}
+
+int pade_pencil_copy(const void *src, void *dst, long long nbytes, void *stream)
+{
+    if (!src || !dst) return fail(400, "null array");
+    if (nbytes < 0) return fail(400, "negative size");
+    hipError_t e = hipMemcpyAsync(dst, src, (size_t)nbytes, hipMemcpyDeviceToDevice, (hipStream_t)stream);
+    return e == hipSuccess ? 0 : hip_fail(e, "pencil copy");
+}
+
+int pade_permute3(const void *src, void *dst, int elem_size, int n1, int n2, int n3, int perm, void *stream)
+{
+    if (!src || !dst) return fail(400, "null array");
+    if (elem_size != 4 && elem_size != 8) return fail(400, "elem_size must be 4 or 8");
+    if (n1 < 1 || n2 < 1 || n3 < 1) return fail(400, "sizes must be positive");
+    if (perm < 1 || perm > 3) return fail(400, "perm must be 1, 2 or 3");
+    hipError_t e = pade::permute_launch(src, dst, elem_size, n1, n2, n3, perm, (hipStream_t)stream);
+    return e == hipSuccess ? 0 : hip_fail(e, "permute");
+}
+
+int pade_plan_export(int kind, int oper, int n, double dx, int periodic, int bc1, int bcn, int adjoint,
+                     int M, double *tab, long long tab_len, int *meta)
+{
+    if (!meta) return fail(400, "null meta");
+    if (kind < pade::CD06 || kind > pade::CF90) return fail(400, "unknown operator kind");
+    if (!oper_valid(kind, oper)) return fail(21, "operation not provided for this operator kind");
+    if (int ierr = pade::init_error(kind, n, periodic != 0)) return fail(ierr, "grid too small");
+    if (n < 4) return fail(21, "no table for n < 4");
+    pade::OpMatrices m;
+    if (int rc = operator_matrices(kind, oper, n, kind == pade::CF90 ? 1.0 : dx, periodic != 0, bc1, bcn,
+                                   adjoint != 0, m))
+        return fail(rc, "operator construction failed");
+    if (M == 0) {
+        M = n;
+        for (int c : {64, 32, 16})
+            if (n % c == 0 && n / c <= 8) { M = c; break; }
+    }
+    std::vector<double> t;
+    Variant v;
+    if (int rc = build_variant(m, M, t, v)) return fail(rc == 1 ? 400 : rc, "factorisation failed");
+    const int vals[12] = {n, v.M, v.P, periodic ? 1 : 0, v.nbl, v.nbr, v.off_bl, v.off_br,
+                          v.off_lu, v.off_S, v.off_Q, v.len};
+    std::memcpy(meta, vals, sizeof(vals));
+    if (tab) {
+        if (tab_len < v.len) return fail(400, "table buffer too small");
+        std::copy(t.begin(), t.end(), tab);
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,72 @@
+// Host-side planning for the compact-difference operators.
+//
+// An operator instance is written as  A y = B f  with A a band matrix of
+// half-width 2 (cyclic when periodic; cd06's tridiagonal A has zero outer
+// bands) and B a 9-point stencil plus explicit boundary rows.  The planner
+// turns (kind, operation, n, dx, periodicity, bc) into
+//   * the stencil and boundary-row tables of B (or of B^T for the adjoint),
+//   * the factors of a partitioned ("SPIKE") solve of A (or A^T): P
+//     partitions of M rows, each with its own band LU, coupling columns S_p
+//     and the rows Q_p of the inverse reduced system,
+// everything in FP64.  The device kernels consume these tables.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pade {
+
+enum Kind { CD06 = 1, CD10 = 2, CF90 = 3 };
+enum Oper { D1 = 1, D2 = 2, FILTER = 3 };
+
+constexpr int kBandW = 2;      // band half-width handled by the solver
+constexpr int kStencilH = 4;   // max stencil half-width (cf90)
+constexpr int kBRows = 8;      // max special boundary rows (B^T: closure rows + H)
+constexpr int kBCols = 16;     // max columns of a special boundary row
+constexpr int kLuStride = 5;   // l1 l2 ip u1 u2 per row
+constexpr int kSStride = 4;    // S columns: tail(p-1)[0..1], head(p+1)[0..1]
+
+// Dense description of one operator: A band rows and B = stencil + boundary rows.
+struct OpMatrices {
+    int n = 0;
+    bool periodic = true;
+    // band[i][o+2] = A(i, i+o) (indices wrapped when periodic)
+    std::vector<double> band;            // n * 5
+    double stencil[9] = {0};             // B interior, offsets -4..4 (1/dx^p folded in)
+    int H = 0;                           // stencil half-width actually used
+    int nbl = 0, nbr = 0;                // special rows at the left / right end
+    // left rows q: out[q] = sum_c bl[q][c] * in[c];
+    // right rows q: out[n-1-q] = sum_c br[q][c] * in[n-1-c]
+    double bl[kBRows][kBCols] = {{0}};
+    double br[kBRows][kBCols] = {{0}};
+};
+
+// Partitioned solve factors (all FP64 on the host).
+struct SpikeFactors {
+    int n = 0, M = 0, P = 0;
+    bool periodic = true;
+    std::vector<double> lu;   // P * M * 5
+    std::vector<double> S;    // P * M * 4
+    std::vector<double> Q;    // P * 4 * (4P)
+};
+
+// Build A and B of the forward operator.  For cd06 bounded, bc1/bcn are the
+// init-time codes (they only choose the LHS closure); otherwise the apply-time
+// codes.  Returns 0 or an error code (324 for a bad bc, 21 unsupported).
+int build_operator(int kind, int oper, int n, double dx, bool periodic, int bc1, int bcn,
+                   OpMatrices &out);
+
+// B^T and A^T of an operator (adjoint).
+void transpose_operator(const OpMatrices &fwd, OpMatrices &adj);
+
+// Factor A for a partitioned solve with partition size M (n % M == 0).
+int spike_factor(const OpMatrices &op, int M, SpikeFactors &out);
+
+// Error code init() returns for this size (0 ok, 2/3/7 as the reference).
+int init_error(int kind, int n, bool periodic);
+
+// Dense D = A^{-1} B (row-major n x n), FP64, for the MFMA form.
+int dense_operator(const OpMatrices &op, std::vector<double> &D);
+
+}  // namespace pade

@@ -1,0 +1,237 @@
+"""Host-side mirror of the reference's operator types ``cd06``, ``cd10``, ``cf90``.
+
+Same procedure names, argument meaning and error behaviour as
+``src/derivatives/cd06.F90``, ``src/derivatives/cd10.F90`` and
+``src/filters/cf90.F90``; the arithmetic runs in ``libpadeops_hip.so``.
+
+Arrays are Fortran-ordered (i fastest): ``dd1`` takes ``f(n, na, nb)``,
+``dd2`` ``f(na, n, nb)``, ``dd3`` ``f(na, nb, n)``.  Accepted arrays:
+
+* torch tensors on the GPU, float64 or float32, whose memory is one dense
+  block in Fortran order (a tensor of shape (n1, n2, n3) with strides
+  (1, n1, n1*n2), e.g. from :func:`fortran_empty`, or any contiguous tensor
+  holding the same elements);
+* NumPy float64 arrays in host memory (copied to the device and back, the
+  ``pade_op_apply_host`` path).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import PADE_CD06, PADE_CD10, PADE_CF90, PADE_D1, PADE_D2, PADE_FILTER, PadeError
+
+try:  # torch is plumbing for device memory and streams
+    import torch
+except ImportError:  # pragma: no cover
+    torch = None
+
+
+def fortran_empty(shape, dtype=None, device="cuda"):
+    """Uninitialised tensor of `shape` (n1, n2, n3) with Fortran strides."""
+    n1, n2, n3 = shape
+    return torch.empty((n3, n2, n1), dtype=dtype or torch.float64, device=device).permute(2, 1, 0)
+
+
+def as_fortran(t):
+    """Copy of a torch tensor laid out in Fortran order (same shape)."""
+    out = fortran_empty(tuple(t.shape), dtype=t.dtype, device=t.device)
+    out.copy_(t)
+    return out
+
+
+def _is_dense_block(t):
+    if t.is_contiguous():
+        return True
+    return t.permute(*reversed(range(t.dim()))).is_contiguous()
+
+
+def _stream_ptr():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _check_device_arrays(f, df, count):
+    if f.dtype != df.dtype:
+        raise TypeError("f and df must have the same dtype")
+    if f.dtype not in (torch.float64, torch.float32):
+        raise TypeError("f must be float64 or float32")
+    if not (f.is_cuda and df.is_cuda):
+        raise TypeError("device arrays must live on the GPU")
+    if f.numel() != count or df.numel() != count:
+        raise ValueError(f"array size {f.numel()} does not match n*na*nb = {count}")
+    if not (_is_dense_block(f) and _is_dense_block(df)):
+        raise ValueError("arrays must be dense blocks in Fortran order")
+    if f.data_ptr() == df.data_ptr():
+        raise ValueError("f and df must not alias")
+
+
+class _CompactOp:
+    """Common plumbing: one native handle per initialised operator."""
+
+    _kind = 0
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        self._n = 0
+        self._periodic = True
+
+    # -- lifecycle -----------------------------------------------------------
+    def _create(self, n, dx, periodic, bc1, bcn):
+        lib = _lib.load()
+        self.destroy()
+        h = ctypes.c_void_p()
+        rc = lib.pade_op_create(self._kind, int(n), float(dx), int(bool(periodic)), int(bc1), int(bcn),
+                                ctypes.byref(h))
+        if rc in (2, 3, 7):      # the reference's init ierr codes
+            return rc
+        _lib.check(rc)
+        self._h = h
+        self._n = int(n)
+        self._periodic = bool(periodic)
+        return 0
+
+    def destroy(self):
+        if self._h:
+            _lib.load().pade_op_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def GetSize(self):
+        return self._n
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- application -----------------------------------------------------------
+    def _apply(self, oper, axis, f, df, na, nb, bc1=None, bcn=None, adjoint=False):
+        if not self._h:
+            raise PadeError(400, "operator used before init")
+        b1 = 0 if bc1 is None else int(bc1)
+        bn = 0 if bcn is None else int(bcn)
+        lib = _lib.load()
+        count = self._n * int(na) * int(nb)
+        if isinstance(f, np.ndarray):
+            if adjoint:
+                raise TypeError("the adjoint is provided for device arrays")
+            if f.dtype != np.float64 or df.dtype != np.float64:
+                raise TypeError("host arrays must be float64")
+            if f.size != count or df.size != count:
+                raise ValueError(f"array size does not match n*na*nb = {count}")
+            fa = np.asfortranarray(f) if f.ndim > 1 else np.ascontiguousarray(f)
+            if not (df.flags.f_contiguous or df.ndim == 1):
+                raise ValueError("df must be a Fortran-ordered array")
+            dp = ctypes.POINTER(ctypes.c_double)
+            _lib.check(lib.pade_op_apply_host(self._h, oper, axis, fa.ctypes.data_as(dp),
+                                              df.ctypes.data_as(dp), int(na), int(nb), b1, bn))
+            return df
+        _check_device_arrays(f, df, count)
+        if f.dtype == torch.float64:
+            fn = lib.pade_op_apply_adjoint if adjoint else lib.pade_op_apply
+        else:
+            fn = lib.pade_op_apply_adjoint_f32 if adjoint else lib.pade_op_apply_f32
+        _lib.check(fn(self._h, oper, axis, ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(df.data_ptr()),
+                      int(na), int(nb), b1, bn, _stream_ptr()))
+        return df
+
+    def dense_apply(self, oper, axis, f, df, na, nb, prec, bc1=None, bcn=None):
+        """Dense-operator MFMA form (precision study); f/df float64 or float32."""
+        lib = _lib.load()
+        count = self._n * int(na) * int(nb)
+        if f.numel() != count or df.numel() != count:
+            raise ValueError("array size does not match n*na*nb")
+        dt = {torch.float64: _lib.PADE_F64, torch.float32: _lib.PADE_F32}
+        _lib.check(lib.pade_op_dense_apply(self._h, oper, axis, ctypes.c_void_p(f.data_ptr()), dt[f.dtype],
+                                           ctypes.c_void_p(df.data_ptr()), dt[df.dtype], int(na), int(nb),
+                                           0 if bc1 is None else int(bc1), 0 if bcn is None else int(bcn),
+                                           int(prec), _stream_ptr()))
+        return df
+
+
+class cd06(_CompactOp):
+    """6th-order compact first derivative (src/derivatives/cd06.F90:70-119).
+
+    ``dd1/dd2/dd3`` take no boundary codes (cd06.F90:775-839).  The second
+    derivative ``d2d1/d2d2/d2d3`` has no reference implementation
+    (cd06.F90:330-343,492-513,745-773,841-853); this library provides the
+    periodic sixth-order tridiagonal scheme (alpha = 2/11) and, when bounded,
+    the first derivative applied twice.
+    """
+
+    _kind = PADE_CD06
+
+    def init(self, n, dx, periodic, bc1=0, bcn=0):
+        """Returns ierr: 3 periodic n < 6, 2 bounded n < 8 (cd06.F90:129-201)."""
+        return self._create(n, dx, periodic, bc1, bcn)
+
+    def dd1(self, f, df, na, nb):
+        return self._apply(PADE_D1, 1, f, df, na, nb)
+
+    def dd2(self, f, df, na, nb):
+        return self._apply(PADE_D1, 2, f, df, na, nb)
+
+    def dd3(self, f, df, na, nb):
+        return self._apply(PADE_D1, 3, f, df, na, nb)
+
+    def d2d1(self, f, df, na, nb):
+        return self._apply(PADE_D2, 1, f, df, na, nb)
+
+    def d2d2(self, f, df, na, nb):
+        return self._apply(PADE_D2, 2, f, df, na, nb)
+
+    def d2d3(self, f, df, na, nb):
+        return self._apply(PADE_D2, 3, f, df, na, nb)
+
+
+class cd10(_CompactOp):
+    """10th-order compact first and second derivatives (src/derivatives/cd10.F90:108-183)."""
+
+    _kind = PADE_CD10
+
+    def init(self, n, dx, periodic, bc1=0, bcn=0):
+        """Returns ierr: 2 for 1 < n < 8 (cd10.F90:195-315)."""
+        return self._create(n, dx, periodic, bc1, bcn)
+
+    def dd1(self, f, df, na, nb, bc1=None, bcn=None):
+        return self._apply(PADE_D1, 1, f, df, na, nb, bc1, bcn)
+
+    def dd2(self, f, df, na, nb, bc1=None, bcn=None):
+        return self._apply(PADE_D1, 2, f, df, na, nb, bc1, bcn)
+
+    def dd3(self, f, df, na, nb, bc1=None, bcn=None):
+        return self._apply(PADE_D1, 3, f, df, na, nb, bc1, bcn)
+
+    def d2d1(self, f, df, na, nb, bc1=None, bcn=None):
+        return self._apply(PADE_D2, 1, f, df, na, nb, bc1, bcn)
+
+    def d2d2(self, f, df, na, nb, bc1=None, bcn=None):
+        return self._apply(PADE_D2, 2, f, df, na, nb, bc1, bcn)
+
+    def d2d3(self, f, df, na, nb, bc1=None, bcn=None):
+        return self._apply(PADE_D2, 3, f, df, na, nb, bc1, bcn)
+
+
+class cf90(_CompactOp):
+    """8th-order compact filter with 90% truncation (src/filters/cf90.F90:54-101)."""
+
+    _kind = PADE_CF90
+
+    def init(self, n, periodic):
+        """Returns ierr: 7 for 1 < n < 10 (cf90.F90:107-175)."""
+        return self._create(n, 1.0, periodic, 0, 0)
+
+    def filter1(self, f, df, na, nb, bc1=None, bcn=None):
+        return self._apply(PADE_FILTER, 1, f, df, na, nb, bc1, bcn)
+
+    def filter2(self, f, df, na, nb, bc1=None, bcn=None):
+        return self._apply(PADE_FILTER, 2, f, df, na, nb, bc1, bcn)
+
+    def filter3(self, f, df, na, nb, bc1=None, bcn=None):
+        return self._apply(PADE_FILTER, 3, f, df, na, nb, bc1, bcn)

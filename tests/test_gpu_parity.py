@@ -1,0 +1,209 @@
+"""GPU parity: every operator through the C ABI against the CPU oracle.
+
+Bar: FP64 results within 1e-12 normwise relative (max|diff| / max|ref|) of
+the dense oracle (oracle/pade_dense.py); FP32 banded within 2e-4 (cf90 has
+cond(A) ~ 2e3); adjoint against the explicit transpose D^T.
+"""
+import itertools
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import pade_dense as pd
+import padeops_amd as po
+from padeops_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+TOL64 = 1e-12
+
+OPS = [("cd06", "d1"), ("cd06", "d2"), ("cd10", "d1"), ("cd10", "d2"), ("cf90", "filter")]
+BCS = [(a, b) for a in (0, 1, -1) for b in (0, 1, -1)]
+
+
+def make(kind, n, dx, periodic, bc1=0, bcn=0):
+    op = {"cd06": po.cd06, "cd10": po.cd10, "cf90": po.cf90}[kind]()
+    ierr = op.init(n, periodic) if kind == "cf90" else op.init(n, dx, periodic, bc1, bcn)
+    assert ierr == 0
+    return op
+
+
+def run(op, kind, which, axis, f, bc=None, adjoint=False):
+    """Apply on the GPU; f is a Fortran-ordered numpy array."""
+    shape = f.shape
+    ft = po.as_fortran(torch.tensor(f, device=DEV))
+    out = po.fortran_empty(shape, dtype=ft.dtype, device=DEV)
+    n = shape[axis - 1]
+    na, nb = [s for i, s in enumerate(shape) if i != axis - 1]
+    oper = {"d1": _lib.PADE_D1, "d2": _lib.PADE_D2, "filter": _lib.PADE_FILTER}[which]
+    b1, bn = (None, None) if bc is None else bc
+    op._apply(oper, axis, ft, out, na, nb, b1, bn, adjoint=adjoint)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def ref_apply(kind, which, f, axis, dx, periodic, bc1, bcn, adjoint=False):
+    n = f.shape[axis - 1]
+    D = pd.dense_operator(kind, which, n, dx, periodic, bc1, bcn)
+    return pd.apply_axis(D.T if adjoint else D, f, axis - 1)
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def field(shape, seed):
+    rng = np.random.default_rng(seed)
+    return np.asfortranarray(rng.standard_normal(shape))
+
+
+def shape_for(axis, n, na=20, nb=7):
+    s = [na, nb]
+    s.insert(axis - 1, n)
+    return tuple(s)
+
+
+@pytest.mark.parametrize("kind,which", OPS)
+@pytest.mark.parametrize("n", [16, 32, 64, 96, 128, 256])
+@pytest.mark.parametrize("axis", [1, 2, 3])
+def test_periodic_fp64(kind, which, n, axis):
+    dx = 2 * np.pi / n
+    f = field(shape_for(axis, n, 70, 5), seed=n + axis)
+    op = make(kind, n, dx, True)
+    out = run(op, kind, which, axis, f)
+    ref = ref_apply(kind, which, f, axis, dx, True, 0, 0)
+    assert relerr(out, ref) < TOL64, (kind, which, n, axis, relerr(out, ref))
+
+
+@pytest.mark.parametrize("kind,which", [("cd10", "d1"), ("cd10", "d2"), ("cf90", "filter")])
+@pytest.mark.parametrize("bc", BCS)
+@pytest.mark.parametrize("n,axis", [(64, 1), (64, 2), (64, 3), (32, 1), (128, 3), (48, 2)])
+def test_bounded_fp64(kind, which, bc, n, axis):
+    dx = 0.7 / (n - 1)
+    f = field(shape_for(axis, n, 66, 3), seed=7 * n + axis)
+    op = make(kind, n, dx, False)
+    out = run(op, kind, which, axis, f, bc=bc)
+    ref = ref_apply(kind, which, f, axis, dx, False, *bc)
+    assert relerr(out, ref) < TOL64, (kind, which, bc, n, axis, relerr(out, ref))
+
+
+@pytest.mark.parametrize("init_bc", [(0, 0), (1, 1), (0, 1)])
+@pytest.mark.parametrize("axis", [1, 2, 3])
+def test_cd06_bounded(init_bc, axis):
+    n = 64
+    dx = 1.0 / (n - 1)
+    f = field(shape_for(axis, n, 64, 4), seed=axis)
+    op = make("cd06", n, dx, False, *init_bc)
+    out = run(op, "cd06", "d1", axis, f)
+    ref = ref_apply("cd06", "d1", f, axis, dx, False, *init_bc)
+    assert relerr(out, ref) < TOL64
+    # bounded second derivative = first derivative applied twice
+    out2 = run(op, "cd06", "d2", axis, f)
+    ref2 = ref_apply("cd06", "d1", ref, axis, dx, False, *init_bc)
+    assert relerr(out2, ref2) < TOL64
+
+
+@pytest.mark.parametrize("kind,which", OPS)
+@pytest.mark.parametrize("n", [33, 24, 10, 200])
+def test_generic_sizes(kind, which, n):
+    """Sizes outside the register path (n not a multiple of 16, or P > 8)."""
+    if kind == "cd06" and n < 6:
+        pytest.skip("below cd06 minimum")
+    per_list = [True] if (kind == "cd06" and which == "d2") else [True, False]
+    for periodic in per_list:
+        if kind == "cf90" and n < 10:
+            continue
+        dx = 0.1
+        for axis in (1, 2, 3):
+            f = field(shape_for(axis, n, 9, 5), seed=n * axis)
+            bc = (1, -1) if (not periodic and kind != "cd06") else (0, 0)
+            op = make(kind, n, dx, periodic)
+            out = run(op, kind, which, axis, f, bc=bc if kind != "cd06" else None)
+            ref = ref_apply(kind, which, f, axis, dx, periodic, *(bc if kind != "cd06" else (0, 0)))
+            assert relerr(out, ref) < TOL64, (kind, which, n, periodic, axis, relerr(out, ref))
+
+
+@pytest.mark.parametrize("kind,which", OPS)
+@pytest.mark.parametrize("periodic", [True, False])
+@pytest.mark.parametrize("n", [32, 64, 40])
+def test_adjoint_fp64(kind, which, periodic, n):
+    if kind == "cd06" and which == "d2" and not periodic:
+        pytest.skip("composition; covered through d1")
+    dx = 0.05
+    bcs = [(0, 0)] if (periodic or kind == "cd06") else [(0, 0), (1, -1), (-1, 1)]
+    for bc in bcs:
+        for axis in (1, 2, 3):
+            g = field(shape_for(axis, n, 65, 3), seed=axis + n)
+            op = make(kind, n, dx, periodic)
+            out = run(op, kind, which, axis, g, bc=bc if kind != "cd06" else None, adjoint=True)
+            ref = ref_apply(kind, which, g, axis, dx, periodic, *bc, adjoint=True)
+            assert relerr(out, ref) < TOL64, (kind, which, periodic, bc, n, axis, relerr(out, ref))
+
+
+@pytest.mark.parametrize("kind,which", OPS)
+@pytest.mark.parametrize("axis", [1, 2, 3])
+def test_fp32_banded(kind, which, axis):
+    n = 128
+    dx = 2 * np.pi / n
+    f = field(shape_for(axis, n, 64, 6), seed=3 + axis)
+    op = make(kind, n, dx, True)
+    ft = po.as_fortran(torch.tensor(f, device=DEV, dtype=torch.float32))
+    out = po.fortran_empty(f.shape, dtype=torch.float32, device=DEV)
+    na, nb = [s for i, s in enumerate(f.shape) if i != axis - 1]
+    oper = {"d1": 1, "d2": 2, "filter": 3}[which]
+    op._apply(oper, axis, ft, out, na, nb)
+    ref = ref_apply(kind, which, f.astype(np.float32).astype(np.float64), axis, dx, True, 0, 0)
+    e = relerr(out.cpu().numpy().astype(np.float64), ref)
+    assert e < (1e-3 if kind == "cf90" else 1e-4), e
+
+
+def test_n1_and_errors():
+    op = po.cd10()
+    assert op.init(1, 0.1, True) == 0
+    f = po.as_fortran(torch.ones((1, 4, 4), dtype=torch.float64, device=DEV))
+    out = po.fortran_empty((1, 4, 4), device=DEV)
+    op.dd1(f, out, 4, 4)
+    assert torch.count_nonzero(out).item() == 0
+    fl = po.cf90()
+    assert fl.init(1, True) == 0
+    fl.filter1(f, out, 4, 4)
+    assert torch.equal(out, f)
+    assert po.cd10().init(5, 0.1, True) == 2
+    assert po.cd06().init(5, 0.1, True) == 3
+    assert po.cd06().init(7, 0.1, False) == 2
+    assert po.cf90().init(9, False) == 7
+    op = make("cd10", 16, 0.1, False)
+    f = po.as_fortran(torch.randn((16, 4, 4), dtype=torch.float64, device=DEV))
+    out = po.fortran_empty((16, 4, 4), device=DEV)
+    with pytest.raises(po.PadeError) as e:
+        op.dd1(f, out, 4, 4, bc1=2, bcn=0)
+    assert e.value.code == 324
+
+
+def test_host_arrays_match_device():
+    n = 64
+    f = field((n, 8, 6), seed=11)
+    op = make("cd10", n, 0.1, True)
+    out = np.zeros_like(f, order="F")
+    op.dd1(f, out, 8, 6)
+    ref = ref_apply("cd10", "d1", f, 1, 0.1, True, 0, 0)
+    assert relerr(out, ref) < TOL64
+
+
+@pytest.mark.parametrize("perm", [1, 2, 3])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_permute_bitwise(perm, dtype):
+    import ctypes
+    n1, n2, n3 = 37, 70, 19
+    x = torch.randn((n3, n2, n1), dtype=dtype, device=DEV)   # memory of Fortran (n1,n2,n3)
+    y = torch.empty_like(x)
+    lib = _lib.load()
+    _lib.check(lib.pade_permute3(ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                 x.element_size(), n1, n2, n3, perm, None))
+    torch.cuda.synchronize()
+    xf = x.cpu().numpy().transpose(2, 1, 0)                   # (n1, n2, n3)
+    exp = {1: xf.transpose(1, 0, 2), 2: xf.transpose(2, 1, 0), 3: xf.transpose(0, 2, 1)}[perm]
+    got = y.cpu().numpy().reshape(-1)
+    assert np.array_equal(got, np.asfortranarray(exp).reshape(-1, order="F"))

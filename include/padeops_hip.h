@@ -103,9 +103,16 @@ int pade_permute3(const void *src, void *dst, int elem_size, int n1, int n2, int
 
 /* Host-only planning export (no GPU needed): writes the FP64 table the
  * kernels consume for (kind, oper, n, dx, periodic, bc1, bcn, adjoint) with
- * partition size M (M = 0: library choice), and meta[12] =
- * {n, M, P, periodic, nbl, nbr, off_bl, off_br, off_lu, off_S, off_Q, len}.
- * Pass tab = NULL to query len.  Used by tests to check the planner on CPU. */
+ * partition size M (M = 0: library choice), and meta[32] =
+ * {n, M, P, periodic, nbl, nbr, off_bl, off_br, off_coef, coef_len, ncls, kq,
+ *  len, 0, 0, 0, cls[0..15]}: stencil at [0,9), left/right boundary rows
+ * (8 x 16) at off_bl/off_br, then at off_coef ncls blocks of 10*M values
+ * (M band-LU rows l1, l2, 1/pivot, u1/pivot, u2/pivot, 0 and M coupling rows
+ * of 4) followed by the reduced-system rows (periodic: the 4 rows of 4P values
+ * of partition 0, which partition p uses rotated by p; bounded: 4 rows for
+ * every partition); partition p uses block cls[p]; kq is the reduced-system
+ * window.  Pass tab = NULL to query len.  Used by tests to check the planner
+ * on CPU. */
 int pade_plan_export(int kind, int oper, int n, double dx, int periodic, int bc1, int bcn,
                      int adjoint, int M, double *tab, long long tab_len, int *meta);
 

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -33,12 +34,13 @@ int hip_fail(hipError_t e, const char *where)
 }
 
 // One factor variant (partition size M) of one plan, as laid out in the
-// device table.
+// device table (see pade_kernels.h).
 struct Variant {
     int M = 0, P = 0;
     long long off = 0;  // element offset of this variant's table in the plan buffer
     int nbl = 0, nbr = 0;
-    int off_bl = 0, off_br = 0, off_lu = 0, off_S = 0, off_Q = 0;
+    int coef_len = 0, ncls = 0, kq = 0;
+    int cls[pade::kMaxPartitions] = {0};
     int len = 0;
 };
 
@@ -51,7 +53,7 @@ struct PlanKey {
 };
 
 struct Plan {
-    std::vector<Variant> spike;  // register-path variants (M in {16,32,64})
+    std::vector<Variant> spike;  // register-path variants (M in {16,32,64}), ascending M
     Variant generic;             // whole pencil as one partition
     bool has_generic = false;
     pade::OpMatrices mats;       // for the dense form
@@ -62,29 +64,71 @@ int build_variant(const pade::OpMatrices &m, int M, std::vector<double> &tab, Va
 {
     pade::SpikeFactors sf;
     if (int rc = pade::spike_factor(m, M, sf)) return rc;
+    const int P = sf.P;
+    const bool periodic = m.periodic;
+    const int m4 = 4 * P;
     v.M = M;
-    v.P = sf.P;
+    v.P = P;
     v.nbl = m.nbl;
     v.nbr = m.nbr;
-    v.off_bl = 16;
-    v.off_br = v.off_bl + pade::kBRowsDev * pade::kBColsDev;
-    v.off_lu = v.off_br + pade::kBRowsDev * pade::kBColsDev;
-    v.off_S = v.off_lu + int(sf.lu.size());
-    v.off_Q = v.off_S + int(sf.S.size());
-    v.len = v.off_Q + int(sf.Q.size());
-    v.len = (v.len + 31) & ~31;
+    // coefficient classes: partitions with bit-identical (LU, S) blocks share one
+    std::vector<int> reps;
+    auto same = [&](int a, int b) {
+        for (int r = 0; r < M; r++) {
+            for (int c = 0; c < 5; c++)
+                if (sf.lu[((size_t)a * M + r) * 5 + c] != sf.lu[((size_t)b * M + r) * 5 + c]) return false;
+            for (int c = 0; c < 4; c++)
+                if (sf.S[((size_t)a * M + r) * 4 + c] != sf.S[((size_t)b * M + r) * 4 + c]) return false;
+        }
+        return true;
+    };
+    for (int p = 0; p < P; p++) {
+        int c = -1;
+        for (int k = 0; k < (int)reps.size(); k++)
+            if (same(reps[k], p)) { c = k; break; }
+        if (c < 0) { c = (int)reps.size(); reps.push_back(p); }
+        if (p < pade::kMaxPartitions) v.cls[p] = c;
+    }
+    if (periodic && reps.size() != 1) return 3;  // periodic blocks are identical by construction
+    v.ncls = (int)reps.size();
+    v.coef_len = v.ncls * 10 * M + (periodic ? 4 * m4 : P * 4 * m4);
+    v.len = (pade::kOffCoef + v.coef_len + 15) & ~15;
+    // reduced-system window: farthest partition whose weight is not negligible
+    double qmax = 0.0;
+    for (double x : sf.Q) qmax = std::max(qmax, std::fabs(x));
+    v.kq = 0;
+    for (int p = 0; p < P; p++)
+        for (int c = 0; c < 4; c++)
+            for (int j = 0; j < m4; j++) {
+                const double x = std::fabs(sf.Q[((size_t)p * 4 + c) * m4 + j]);
+                if (x <= 1e-18 * qmax) continue;
+                int d = std::abs(j / 4 - p);
+                if (periodic) d = std::min(d, P - d);
+                v.kq = std::max(v.kq, d);
+            }
     v.off = (long long)tab.size();
     tab.resize(tab.size() + v.len, 0.0);
     double *t = &tab[v.off];
     for (int k = 0; k < 9; k++) t[k] = m.stencil[k];
     for (int r = 0; r < pade::kBRowsDev; r++)
         for (int c = 0; c < pade::kBColsDev; c++) {
-            t[v.off_bl + r * pade::kBColsDev + c] = m.bl[r][c];
-            t[v.off_br + r * pade::kBColsDev + c] = m.br[r][c];
+            t[pade::kOffBl + r * pade::kBColsDev + c] = m.bl[r][c];
+            t[pade::kOffBr + r * pade::kBColsDev + c] = m.br[r][c];
         }
-    std::copy(sf.lu.begin(), sf.lu.end(), t + v.off_lu);
-    std::copy(sf.S.begin(), sf.S.end(), t + v.off_S);
-    std::copy(sf.Q.begin(), sf.Q.end(), t + v.off_Q);
+    double *ca = t + pade::kOffCoef;
+    for (int k = 0; k < v.ncls; k++) {
+        const int p = reps[k];
+        double *blk = ca + (size_t)k * 10 * M;
+        for (int r = 0; r < M; r++) {
+            for (int c = 0; c < 5; c++) blk[r * 6 + c] = sf.lu[((size_t)p * M + r) * 5 + c];
+            for (int c = 0; c < 4; c++) blk[6 * M + r * 4 + c] = sf.S[((size_t)p * M + r) * 4 + c];
+        }
+    }
+    double *qa = ca + (size_t)v.ncls * 10 * M;
+    const int nq = periodic ? 1 : P;
+    for (int p = 0; p < nq; p++)
+        for (int c = 0; c < 4; c++)
+            for (int j = 0; j < m4; j++) qa[((size_t)p * 4 + c) * m4 + j] = sf.Q[((size_t)p * 4 + c) * m4 + j];
     return 0;
 }
 
@@ -148,7 +192,7 @@ int build_all_plans(pade_op_s *op)
                                                adj != 0, pl.mats))
                     return fail(rc, "operator construction failed");
                 for (int M : {16, 32, 64}) {
-                    if (op->n % M || op->n / M > 8) continue;
+                    if (op->n % M || op->n / M > pade::kMaxPartitions) continue;
                     Variant v;
                     if (build_variant(pl.mats, M, tab, v) == 0) pl.spike.push_back(v);
                 }
@@ -186,20 +230,37 @@ int fill_zero(T *dst, long long count, hipStream_t s)
     return e == hipSuccess ? 0 : hip_fail(e, "fill");
 }
 
-// Choose the variant for this launch: the largest M whose workgroup count
-// keeps the chip busy; the smallest otherwise.
-const Variant *pick_variant(const Plan &pl, long long npencils)
+int env_int(const char *name)
 {
-    if (pl.spike.empty()) return nullptr;
-    const long long groups = (npencils + 63) / 64;
-    const Variant *best = &pl.spike.front();  // smallest M first
+    const char *v = std::getenv(name);
+    return v ? std::atoi(v) : 0;
+}
+
+// Choose (partition size M, partitions per wave PW) for this launch.
+// Two partitions per wave (two workgroups per CU) throughout; x pencils
+// (LDS-transposed, mode 1) prefer M = 64, strided y/z pencils M = 32 -- the
+// fastest variants measured at 512^3 for FP64 and FP32.  Small problems take
+// the smallest M to raise the wave count.  PADE_SPIKE_M / PADE_SPIKE_PW
+// override the choice (tuning).
+const Variant *pick_variant(const Plan &pl, long long npencils, int mode, int *pw_out)
+{
+    static const int envM = env_int("PADE_SPIKE_M"), envPW = env_int("PADE_SPIKE_PW");
+    const Variant *best = nullptr;
+    int best_pw = 1, best_score = -1;
+    const int pref_m = mode == 1 ? 64 : 32;
     for (const Variant &v : pl.spike)
-        if (groups * v.P >= 2048 || v.M == pl.spike.front().M) best = &v;
-    // prefer the largest M that still gives >= 2048 waves
-    const Variant *pick = nullptr;
-    for (const Variant &v : pl.spike)
-        if (groups * v.P >= 2048) pick = &v;
-    return pick ? pick : best;
+        for (int pw : {1, 2}) {
+            if ((v.P + pw - 1) / pw > 8) continue;
+            if (envM && v.M != envM) continue;
+            if (envPW && pw != envPW) continue;
+            int score = (pw == 2 ? 100 : 0) + (v.M == pref_m ? 50 : (v.M == 64 ? 30 : (v.M == 32 ? 20 : 10)));
+            const long long groups = (npencils + 64 / pw - 1) / (64 / pw);
+            const long long waves = groups * ((v.P + pw - 1) / pw);
+            if (waves < 4096) score = 1000 - v.M + (pw == 2 ? 1 : 0);  // latency-bound size: most waves
+            if (score > best_score) { best_score = score; best = &v; best_pw = pw; }
+        }
+    *pw_out = best_pw;
+    return best;
 }
 
 template <typename T>
@@ -226,7 +287,8 @@ int apply_one(pade_op_t op, const Plan &pl, int axis, const T *f, T *df, int na,
         a.A1 = a.npencils; a.B1 = 1; a.B2 = 0; a.stride = (long long)na * nb;
     }
     const T *base = std::is_same<T, double>::value ? (const T *)op->dtab64 : (const T *)op->dtab32;
-    const Variant *v = pick_variant(pl, a.npencils);
+    int pw = 1;
+    const Variant *v = pick_variant(pl, a.npencils, mode, &pw);
     const bool use_generic = (v == nullptr);
     if (use_generic) {
         if (!pl.has_generic) return fail(21, "no solver variant for this size");
@@ -237,23 +299,28 @@ int apply_one(pade_op_t op, const Plan &pl, int axis, const T *f, T *df, int na,
     a.P = v->P;
     a.nbl = v->nbl;
     a.nbr = v->nbr;
-    a.off_bl = v->off_bl;
-    a.off_br = v->off_br;
-    a.off_lu = v->off_lu;
-    a.off_S = v->off_S;
-    a.off_Q = v->off_Q;
+    a.coef_len = v->coef_len;
+    a.ncls = v->ncls;
+    a.kq = v->kq;
+    for (int k = 0; k < pade::kMaxPartitions; k++) a.cls[k] = v->cls[k];
+    // Periodic operators are circulant, so A^T and B^T commute and the adjoint
+    // B^T A^{-T} equals A^{-T} B^T: the transposed tables run through the
+    // forward kernel (stencil first).  This avoids amplifying the Nyquist band
+    // by A^{-T} before B^T damps it again.  Bounded operators use the adjoint
+    // kernel (solve first, then B^T with the transposed closures).
+    const bool kadj = adj && !op->periodic;
     hipError_t e;
     if (!use_generic) {
-        e = pade::launch_spike<T>(a, v->M, adj, mode, s);
+        e = pade::launch_spike<T>(a, v->M, pw, kadj, mode, s);
         if (e != hipSuccess) return hip_fail(e, "spike_apply launch");
         return 0;
     }
     T *work = nullptr;
-    if (adj) {
+    if (kadj) {
         e = hipMallocAsync((void **)&work, sizeof(T) * (size_t)a.npencils * n, s);
         if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(work)");
     }
-    e = pade::launch_generic<T>(a, adj, work, s);
+    e = pade::launch_generic<T>(a, kadj, work, s);
     if (work) hipFreeAsync(work, s);
     if (e != hipSuccess) return hip_fail(e, "generic_apply launch");
     return 0;
@@ -463,14 +530,15 @@ int pade_plan_export(int kind, int oper, int n, double dx, int periodic, int bc1
         return fail(rc, "operator construction failed");
     if (M == 0) {
         M = n;
-        for (int c : {64, 32, 16})
-            if (n % c == 0 && n / c <= 8) { M = c; break; }
+        for (int c : {32, 64, 16})
+            if (n % c == 0 && n / c <= pade::kMaxPartitions) { M = c; break; }
     }
     std::vector<double> t;
     Variant v;
     if (int rc = build_variant(m, M, t, v)) return fail(rc == 1 ? 400 : rc, "factorisation failed");
-    const int vals[12] = {n, v.M, v.P, periodic ? 1 : 0, v.nbl, v.nbr, v.off_bl, v.off_br,
-                          v.off_lu, v.off_S, v.off_Q, v.len};
+    int vals[32] = {n, v.M, v.P, periodic ? 1 : 0, v.nbl, v.nbr, pade::kOffBl, pade::kOffBr,
+                    pade::kOffCoef, v.coef_len, v.ncls, v.kq, v.len};
+    for (int k = 0; k < pade::kMaxPartitions; k++) vals[16 + k] = v.cls[k];
     std::memcpy(meta, vals, sizeof(vals));
     if (tab) {
         if (tab_len < v.len) return fail(400, "table buffer too small");

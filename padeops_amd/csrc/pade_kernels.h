@@ -11,12 +11,21 @@ constexpr int kBRowsDev = 8;
 constexpr int kBColsDev = 16;
 
 // Device table layout of one plan (element offsets):
-//   [0, 16)                stencil (offsets -4..4, 1/dx^p folded in)
-//   [off_bl, +8*16)        left special rows
-//   [off_br, +8*16)        right special rows (stored reflected)
-//   [off_lu, +P*M*5)       per-partition band LU: l1 l2 1/piv u1/piv u2/piv
-//   [off_S,  +P*M*4)       per-partition coupling columns
-//   [off_Q,  +P*4*4P)      per-partition rows of the inverse reduced system
+//   [0, 16)                     stencil (offsets -4..4, 1/dx^p folded in)
+//   [16, 16+128)                left special rows (8 x 16)
+//   [144, 144+128)              right special rows (8 x 16, stored reflected)
+//   [272, 272+coef_len)         coefficient area, copied to LDS as a whole:
+//       ncls blocks of 10M: M band-LU rows (l1, l2, 1/piv, u1/piv, u2/piv, 0)
+//                           then M coupling rows S (4 each);
+//       then the reduced-system rows: periodic -> the 4 rows (4P each) of
+//       partition 0 (partition p uses them rotated by p); bounded -> the 4
+//       rows of every partition ([P][4][4P]).
+// Partition p uses block cls[p].
+constexpr int kOffBl = 16;
+constexpr int kOffBr = 144;
+constexpr int kOffCoef = 272;
+constexpr int kMaxPartitions = 16;
+
 template <typename T>
 struct ApplyArgs {
     const T *f;
@@ -26,13 +35,16 @@ struct ApplyArgs {
     long long A1, B1, B2;  // pencil q -> base offset (q % A1)*B1 + (q / A1)*B2
     long long stride;      // element distance between consecutive points of a pencil
     int n, M, P, periodic, nbl, nbr;
-    int off_bl, off_br, off_lu, off_S, off_Q;
+    int coef_len;          // length of the coefficient area
+    int ncls;              // number of coefficient blocks
+    int kq;                // reduced-system window: partitions p-kq..p+kq (2kq+1 >= P: all)
+    int cls[kMaxPartitions];
 };
 
-size_t spike_lds_bytes(size_t tsize, int P, bool adj, int mode);
+size_t spike_lds_bytes(size_t tsize, int M, int P, int PW, bool adj, int mode, int coef_len);
 
 template <typename T>
-hipError_t launch_spike(const ApplyArgs<T> &a, int M, bool adj, int mode, hipStream_t s);
+hipError_t launch_spike(const ApplyArgs<T> &a, int M, int PW, bool adj, int mode, hipStream_t s);
 
 template <typename T>
 hipError_t launch_generic(const ApplyArgs<T> &a, bool adj, T *work, hipStream_t s);

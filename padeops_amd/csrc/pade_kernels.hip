@@ -6,28 +6,31 @@
 // (cyclic when periodic), B a 9-point stencil with explicit boundary rows.
 //
 // Fast path (spike_apply): a pencil of n = P*M points is split into P
-// partitions of M points.  A workgroup holds 64 pencils; wave p owns partition
-// p of all of them, one pencil per lane, with the M values in registers.
-// Each lane
+// partitions of M points.  A workgroup holds NPEN = 64/PW pencils and all P
+// partitions of them; each wave serves PW partitions (PW = 1: 64 pencils x 1
+// partition, PW = 2: 32 pencils x 2 partitions), one (pencil, partition) per
+// lane with the M values in registers.  Each lane
 //   1. loads its window (M points + 4-point halos) and evaluates the stencil,
-//   2. solves its partition's band block with the block's own LU factors
-//      (coefficients are wave-uniform -> scalar loads),
+//   2. solves its partition's band block (forward + backward sweep) with
+//      coefficients staged once per workgroup in LDS,
 //   3. publishes the four "tip" values (first two / last two) through LDS,
-//   4. forms the two neighbour tips on each side from the precomputed inverse
-//      of the reduced tip system, and
-//   5. corrects its partition with the precomputed coupling columns S_p.
+//   4. forms the neighbouring tips from the precomputed inverse of the reduced
+//      tip system (only the partitions whose weights are not negligible), and
+//   5. corrects its partition with the coupling columns.
 // This is an exact partitioned (SPIKE-type) solve: every value of f is read
-// once and every value of out written once, so HBM traffic is the algorithmic
-// 2 * sizeof(T) bytes per point.
+// once and every value of out written once, so the HBM traffic is the
+// algorithmic 2 * sizeof(T) bytes per point.  PW = 2 halves the pencils per
+// workgroup so that two workgroups fit on a CU and one's memory phase
+// overlaps the other's arithmetic.
 //
-// Pencils along y and z are strided in memory: the 64 lanes of a wave are
-// 64 neighbouring pencils, so every load is a contiguous 64-element row.
-// Pencils along x are contiguous: the wave moves 64 x 16 tiles through LDS so
-// that global loads/stores stay coalesced (MODE 1).
+// Pencils along y and z are strided in memory: lanes are neighbouring pencils,
+// so every load is a contiguous row.  Pencils along x are contiguous: each
+// wave moves small tiles through LDS so that global loads/stores stay
+// coalesced (MODE 1).
 //
 // Generic path (generic_apply): one lane per pencil with the whole pencil as
 // one partition in global memory -- for sizes the register path does not
-// cover (n not a multiple of 16, or too large).
+// cover (n not a multiple of 16, or more than 16 partitions).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -43,6 +46,15 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Workgroup barrier that orders LDS only: outstanding global loads stay in
+// flight across it.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ int wrap_index(int j, int n, int periodic)
 {
     if (j < 0) return periodic ? j + n : 0;
@@ -50,256 +62,363 @@ __device__ __forceinline__ int wrap_index(int j, int n, int periodic)
     return j;
 }
 
-constexpr int kXTile = 16;       // points per LDS transpose chunk (x pencils)
-constexpr int kXPad = 17;        // padded row length of the transpose tile
+// Element at wave-uniform pointer `u` plus a per-lane byte offset `off`:
+// written this way the address is an SGPR base plus one 32-bit VGPR, so the
+// unrolled loads/stores do not each carry a 64-bit per-lane address.
+template <typename T>
+__device__ __forceinline__ T ld(const T *u, unsigned off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(u) + off);
+}
+template <typename T>
+__device__ __forceinline__ void st(T *u, unsigned off, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(u) + off) = v;
+}
 
-template <typename T, int M, bool ADJ, int MODE>
+// Scheduling region length (rows): a compiler memory barrier every kRegion
+// rows of the unrolled sweeps keeps the LDS coefficient reads next to their
+// use; left alone the compiler issues all of them up front and spills them.
+// (Field loads are all issued before the sweeps and stores come after, so the
+// barrier does not reduce memory-level parallelism.)
+constexpr int kRegion = 4;
+__device__ __forceinline__ void mem_region_fence() { asm volatile("" ::: "memory"); }
+// Materialise a value here: keeps MachineSink from moving the arithmetic that
+// produces it down into the (conditional) store block while the coefficient
+// reads it consumed stay behind, alive in spilled registers.
+template <typename T>
+__device__ __forceinline__ void pin(T &x) { asm volatile("" : "+v"(x)); }
+#define PADE_REGION_END(t) \
+    do {                                         \
+        if (((t) % kRegion) == kRegion - 1) mem_region_fence(); \
+    } while (0)
+
+// x-pencil transpose chunk: points per partition per LDS round trip
+template <int PW>
+struct XTile {
+    static constexpr int XT = 8;
+    static constexpr int PAD = XT + 1;
+};
+
+// LDS layout (elements of T):
+//   tips  [4P][NPEN]
+//   halo  [8P][NPEN]               adjoint only
+//   bnd   [256]                    boundary rows (left 128, right 128)
+//   coef  [coef_len]               coefficient classes + reduced-system rows
+//   xt    [waves][64][PAD]         x pencils only
+__host__ __device__ inline int lds_elems(int M, int P, int PW, bool adj, int mode, int coef_len)
+{
+    const int npen = 64 / PW;
+    const int waves = (P + PW - 1) / PW;
+    int e = (adj ? 12 : 4) * P * npen + 256 + coef_len;
+    if (mode == 1) e += waves * 64 * 9;
+    (void)M;
+    return e;
+}
+
+size_t spike_lds_bytes(size_t tsize, int M, int P, int PW, bool adj, int mode, int coef_len)
+{
+    return (size_t)lds_elems(M, P, PW, adj, mode, coef_len) * tsize;
+}
+
+template <typename T, int M>
+__device__ __forceinline__ void boundary_rows(const T (&w)[M + 8], const T *rows, int nrows, bool right,
+                                              T (&res)[kBRowsDev])
+{
+#pragma unroll
+    for (int r = 0; r < kBRowsDev; r++) {
+        T s = T(0);
+        if (r < nrows) {
+#pragma unroll
+            for (int c = 0; c < kBColsDev; c++) s += rows[r * kBColsDev + c] * (right ? w[M + 3 - c] : w[4 + c]);
+        }
+        res[r] = s;
+    }
+}
+
+// B (or B^T) applied in place on the window: w[t+4] is point t on entry,
+// w[t] is row t on exit; special rows at the ends of a bounded pencil replace
+// the interior stencil there.
+template <typename T, int M>
+__device__ __forceinline__ void apply_stencil(T (&w)[M + 8], const T *__restrict__ tab, const T *bnd,
+                                              bool edge_left, bool edge_right, int nbl, int nbr)
+{
+    T lb[kBRowsDev], rb[kBRowsDev];
+    if (edge_left) boundary_rows<T, M>(w, bnd, nbl, false, lb);
+    if (edge_right) boundary_rows<T, M>(w, bnd + 128, nbr, true, rb);
+    T cst[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) cst[k] = tab[k];
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        T s = cst[0] * w[t];
+#pragma unroll
+        for (int k = 1; k < 9; k++) s += cst[k] * w[t + k];
+        w[t] = s;
+    }
+    if (edge_left) {
+#pragma unroll
+        for (int r = 0; r < kBRowsDev; r++)
+            if (r < nbl) w[r] = lb[r];
+    }
+    if (edge_right) {
+#pragma unroll
+        for (int r = 0; r < kBRowsDev; r++)
+            if (r < nbr) w[M - 1 - r] = rb[r];
+    }
+}
+
+// Occupancy target per instantiation (waves per SIMD): with it the compiler
+// schedules against the matching register budget (512 / waves VGPRs) instead
+// of hoisting every coefficient read of the unrolled loops.  FP64 M <= 32
+// fits 128 VGPRs, so two workgroups of 8 waves share a CU.
+template <typename T, int M>
+constexpr int spike_waves_per_eu()
+{
+    return sizeof(T) == 8 ? (M >= 64 ? 2 : 4) : (M >= 64 ? 4 : (M >= 32 ? 4 : 8));
+}
+
+template <typename T, int M, int PW, bool ADJ, int MODE, bool BND>
 __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
 {
+    constexpr int NPEN = 64 / PW;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *smem = reinterpret_cast<T *>(smem_raw);
     const int lane = threadIdx.x & 63;
-    const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int half = PW == 2 ? (lane >> 5) : 0;
+    const int pen = lane & (NPEN - 1);
     const int P = a.P;
     const int n = a.n;
-    const int pM = p * M;
-    const long long q0 = (long long)blockIdx.x * 64;
-    const long long q = q0 + lane;
-    const bool valid = q < a.npencils;
-    const long long base = (q % a.A1) * a.B1 + (q / a.A1) * a.B2;
-    const long long st = a.stride;
+    const int p = wave * PW + half;              // this lane's partition
+    const bool active = p < P;
+    const int pc = active ? p : P - 1;           // clamped for addressing
+    const int pM = pc * M;
+    const int pw0 = wave * PW * M;               // first point of this wave's first partition
+    const long long q0 = (long long)blockIdx.x * NPEN;
+    const long long q = q0 + pen;
+    const bool inrange = q < a.npencils;
+    const bool valid = active && inrange;
+    const long long base0 = (q0 % a.A1) * a.B1 + (q0 / a.A1) * a.B2;
+    const unsigned loff =
+        inrange ? (unsigned)(((q % a.A1) * a.B1 + (q / a.A1) * a.B2 - base0) * (long long)sizeof(T)) : 0u;
+    const long long stp = a.stride;
+    const T *__restrict__ fb = a.f + base0;
+    T *__restrict__ ob = a.out + base0;
     const T *__restrict__ tab = a.tab;
-    const T *__restrict__ lu = tab + a.off_lu + (long long)p * M * 5;
-    const T *__restrict__ S = tab + a.off_S + (long long)p * M * 4;
-    const T *__restrict__ Q = tab + a.off_Q + (long long)p * 4 * (4 * P);
-    const bool edge_left = !a.periodic && p == 0;
-    const bool edge_right = !a.periodic && p == P - 1;
+    const bool edge_left = BND && p == 0;
+    const bool edge_right = BND && p == P - 1;
 
-    T *tips = smem;                          // [4P][64]
-    T *halo = smem + 4 * P * 64;             // [8P][64]   (adjoint)
-    T *xt = smem + (ADJ ? 12 : 4) * P * 64 + p * 64 * kXPad;  // per-wave tile (MODE 1)
+    T *tips = smem;                                     // [4P][NPEN]
+    T *halo = smem + 4 * P * NPEN;                      // [8P][NPEN]  (adjoint)
+    T *bnd = smem + (ADJ ? 12 : 4) * P * NPEN;          // [256]
+    T *coef = bnd + 256;                                // [coef_len]
+    T *xt = coef + a.coef_len + wave * 64 * XTile<PW>::PAD;
+
+    // --------------------------- 0. coefficients -> LDS (ahead of the field)
+    {
+        const T *g = tab + kOffCoef;
+        const int len = a.coef_len;
+        for (int idx = threadIdx.x; idx < len; idx += blockDim.x) coef[idx] = g[idx];
+        if (BND)
+            for (int idx = threadIdx.x; idx < 256; idx += blockDim.x) bnd[idx] = tab[kOffBl + idx];
+    }
 
     T w[M + 8];
 
-    // ---------------------------------------------------------------- load
+    // --------------------------------------------------------------- 1. load
     if constexpr (MODE == 0) {
+        // main rows: uniform base of this wave's first partition + per-lane offset
+        const unsigned hoff = loff + (unsigned)((long long)half * M * stp * (long long)sizeof(T));
         if constexpr (!ADJ) {
 #pragma unroll
-            for (int t = 0; t < M + 8; t++) {
-                const int j = wrap_index(pM - 4 + t, n, a.periodic);
-                w[t] = valid ? a.f[base + (long long)j * st] : T(0);
+            for (int t = 4; t < M + 4; t++) w[t] = ld(fb + (long long)(pw0 - 4 + t) * stp, active ? hoff : loff);
+            // halo rows (wrapped per partition): full per-lane addresses
+            const T *fq = a.f + base0 + (long long)(loff / sizeof(T));
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int jl = wrap_index(pM - 4 + k, n, !BND);
+                const int jr = wrap_index(pM + M + k, n, !BND);
+                w[k] = fq[(long long)jl * stp];
+                w[M + 4 + k] = fq[(long long)jr * stp];
             }
         } else {
 #pragma unroll
-            for (int t = 0; t < M; t++) w[t] = valid ? a.f[base + (long long)(pM + t) * st] : T(0);
+            for (int t = 0; t < M; t++) w[t] = ld(fb + (long long)(pw0 + t) * stp, active ? hoff : loff);
         }
     } else {
-        constexpr int off = ADJ ? 0 : 4;
+        // x pencils: tile rows r = (pencil, half) of this wave, XT points per
+        // chunk.  The forward window [-4, M+4) of each partition (halo
+        // included, wrapped or clamped per element) is (M+8)/XT coalesced
+        // chunks; the adjoint needs only [0, M).
+        constexpr int XT = XTile<PW>::XT, PAD = XTile<PW>::PAD;
+        constexpr int G = 64 / XT;                       // rows per load instruction
+        constexpr int NCH = ADJ ? M / XT : (M + 8) / XT;
+        constexpr int P0 = ADJ ? 0 : -4;                 // first window point
+        static_assert((M + 8) % XT == 0 && M % XT == 0, "chunking");
 #pragma unroll
-        for (int c = 0; c < M / kXTile; c++) {
+        for (int c = 0; c < NCH; c++) {
 #pragma unroll
-            for (int s = 0; s < kXTile; s++) {
-                const int pen = 4 * s + (lane >> 4);
-                const int pt = lane & 15;
-                const long long qq = q0 + pen;
-                T v = T(0);
-                if (qq < a.npencils) v = a.f[qq * n + pM + c * kXTile + pt];
-                xt[pen * kXPad + pt] = v;
+            for (int s = 0; s < XT; s++) {
+                const int r = s * G + lane / XT;             // tile row
+                const int rp = r / PW, rh = r % PW;
+                const int part = wave * PW + rh;
+                const bool ok = q0 + rp < a.npencils && part < P;
+                const int j = wrap_index((ok ? part : 0) * M + P0 + c * XT + (lane % XT), n, !BND);
+                const unsigned xo = ok ? (unsigned)((rp * (long long)n + j) * (long long)sizeof(T)) : 0u;
+                w[c * XT + s] = ld(fb, xo);
             }
-            wave_lds_sync();
-#pragma unroll
-            for (int j = 0; j < kXTile; j++) w[off + c * kXTile + j] = xt[lane * kXPad + j];
-            wave_lds_sync();
         }
-        if constexpr (!ADJ) {
+        const int myrow = pen * PW + half;
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int jl = wrap_index(pM - 4 + k, n, a.periodic);
-                const int jr = wrap_index(pM + M + k, n, a.periodic);
-                w[k] = valid ? a.f[base + jl] : T(0);
-                w[M + 4 + k] = valid ? a.f[base + jr] : T(0);
-            }
+        for (int c = 0; c < NCH; c++) {
+#pragma unroll
+            for (int s = 0; s < XT; s++) xt[(s * G + lane / XT) * PAD + (lane % XT)] = w[c * XT + s];
+            wave_lds_sync();
+#pragma unroll
+            for (int j = 0; j < XT; j++) w[c * XT + j] = xt[myrow * PAD + j];
+            wave_lds_sync();
         }
     }
 
-    T lb[kBRowsDev], rb[kBRowsDev];
-    if constexpr (!ADJ) {
-        // ------------------------------------------------------- B f (stencil)
-        if (edge_left) {
-            const T *bl = tab + a.off_bl;
-#pragma unroll
-            for (int r = 0; r < kBRowsDev; r++) {
-                T s = T(0);
-                if (r < a.nbl) {
-#pragma unroll
-                    for (int c = 0; c < kBColsDev; c++) s += bl[r * kBColsDev + c] * w[4 + c];
-                }
-                lb[r] = s;
-            }
-        }
-        if (edge_right) {
-            const T *br = tab + a.off_br;
-#pragma unroll
-            for (int r = 0; r < kBRowsDev; r++) {
-                T s = T(0);
-                if (r < a.nbr) {
-#pragma unroll
-                    for (int c = 0; c < kBColsDev; c++) s += br[r * kBColsDev + c] * w[M + 3 - c];
-                }
-                rb[r] = s;
-            }
-        }
-        T cst[9];
-#pragma unroll
-        for (int k = 0; k < 9; k++) cst[k] = tab[k];
-#pragma unroll
-        for (int t = 0; t < M; t++) {
-            T s = cst[0] * w[t];
-#pragma unroll
-            for (int k = 1; k < 9; k++) s += cst[k] * w[t + k];
-            w[t] = s;
-        }
-        if (edge_left) {
-#pragma unroll
-            for (int r = 0; r < kBRowsDev; r++)
-                if (r < a.nbl) w[r] = lb[r];
-        }
-        if (edge_right) {
-#pragma unroll
-            for (int r = 0; r < kBRowsDev; r++)
-                if (r < a.nbr) w[M - 1 - r] = rb[r];
-        }
-    }
+    // coefficient blocks of this lane's partition
+    const int cls = BND ? a.cls[pc] : 0;
+    const T *lu = coef + cls * 10 * M;      // [M][6]
+    const T *S = lu + 6 * M;                // [M][4]
+    const T *Qb = coef + a.ncls * 10 * M;   // periodic: [4][4P] of partition 0; bounded: [P][4][4P]
 
-    // -------------------------------------------------- local band LU solve
+    lds_barrier();   // coefficients visible
+
+    if constexpr (!ADJ) apply_stencil<T, M>(w, tab, bnd, edge_left, edge_right, a.nbl, a.nbr);
+
+    // ---------------------------------------------- 2. local band LU solve
 #pragma unroll
     for (int t = 1; t < M; t++) {
-        T z = w[t] - lu[t * 5 + 0] * w[t - 1];
-        if (t >= 2) z -= lu[t * 5 + 1] * w[t - 2];
+        T z = w[t] - lu[t * 6 + 0] * w[t - 1];
+        if (t >= 2) z -= lu[t * 6 + 1] * w[t - 2];
         w[t] = z;
+        PADE_REGION_END(t);
     }
-    w[M - 1] = w[M - 1] * lu[(M - 1) * 5 + 2];
-    w[M - 2] = w[M - 2] * lu[(M - 2) * 5 + 2] - lu[(M - 2) * 5 + 3] * w[M - 1];
+    w[M - 1] = w[M - 1] * lu[(M - 1) * 6 + 2];
+    w[M - 2] = w[M - 2] * lu[(M - 2) * 6 + 2] - lu[(M - 2) * 6 + 3] * w[M - 1];
 #pragma unroll
-    for (int t = M - 3; t >= 0; t--)
-        w[t] = w[t] * lu[t * 5 + 2] - lu[t * 5 + 3] * w[t + 1] - lu[t * 5 + 4] * w[t + 2];
+    for (int t = M - 3; t >= 0; t--) {
+        w[t] = w[t] * lu[t * 6 + 2] - lu[t * 6 + 3] * w[t + 1] - lu[t * 6 + 4] * w[t + 2];
+        PADE_REGION_END(t);
+    }
 
-    // ------------------------------------------- reduced system + correction
-    if (a.periodic || P > 1) {
-        tips[(4 * p + 0) * 64 + lane] = w[0];
-        tips[(4 * p + 1) * 64 + lane] = w[1];
-        tips[(4 * p + 2) * 64 + lane] = w[M - 2];
-        tips[(4 * p + 3) * 64 + lane] = w[M - 1];
+    // --------------------------------- 3-5. reduced system and correction
+    if (!BND || P > 1) {
+        if (active) {
+            tips[(4 * p + 0) * NPEN + pen] = w[0];
+            tips[(4 * p + 1) * NPEN + pen] = w[1];
+            tips[(4 * p + 2) * NPEN + pen] = w[M - 2];
+            tips[(4 * p + 3) * NPEN + pen] = w[M - 1];
+        }
+        // full barrier: the field has been consumed, and it also keeps the
+        // coupling-column reads below from being hoisted above this point
         __syncthreads();
         T u0 = T(0), u1 = T(0), u2 = T(0), u3 = T(0);
         const int m = 4 * P;
-        for (int j = 0; j < m; j++) {
-            const T g = tips[j * 64 + lane];
-            u0 += Q[0 * m + j] * g;
-            u1 += Q[1 * m + j] * g;
-            u2 += Q[2 * m + j] * g;
-            u3 += Q[3 * m + j] * g;
-        }
+        const int kq = a.kq;
+        const bool all = 2 * kq + 1 >= P;
+        const int d0 = all ? 0 : -kq;
+        const int d1 = all ? P - 1 : kq;
+        for (int d = d0; d <= d1; d++) {
+            int qq = all ? d : pc + d;                 // source partition
+            if (!all) {
+                if (qq < 0) { if (BND) continue; qq += P; }
+                if (qq >= P) { if (BND) continue; qq -= P; }
+            }
+            // periodic: the rows of partition pc are those of partition 0
+            // rotated by pc partitions; bounded: each partition has its own
+            int qrel = qq - pc;
+            if (qrel < 0) qrel += P;
+            const T *Qr = BND ? Qb + (long long)pc * 4 * m + 4 * qq : Qb + 4 * qrel;
 #pragma unroll
-        for (int t = 0; t < M; t++)
+            for (int k = 0; k < 4; k++) {
+                const T g = tips[(4 * qq + k) * NPEN + pen];
+                u0 += Qr[0 * m + k] * g;
+                u1 += Qr[1 * m + k] * g;
+                u2 += Qr[2 * m + k] * g;
+                u3 += Qr[3 * m + k] * g;
+            }
+        }
+        mem_region_fence();
+#pragma unroll
+        for (int t = 0; t < M; t++) {
             w[t] -= S[t * 4 + 0] * u0 + S[t * 4 + 1] * u1 + S[t * 4 + 2] * u2 + S[t * 4 + 3] * u3;
+            pin(w[t]);
+            PADE_REGION_END(t);
+        }
     }
 
     if constexpr (ADJ) {
         // --------------------------------- B^T of the solved values (halo via LDS)
-        const bool exch = a.periodic || P > 1;
+        const bool exch = !BND || P > 1;
         if (exch) {
+            if (active) {
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                halo[(8 * p + k) * 64 + lane] = w[k];
-                halo[(8 * p + 4 + k) * 64 + lane] = w[M - 4 + k];
+                for (int k = 0; k < 4; k++) {
+                    halo[(8 * p + k) * NPEN + pen] = w[k];
+                    halo[(8 * p + 4 + k) * NPEN + pen] = w[M - 4 + k];
+                }
             }
             __syncthreads();
         }
-        int pl = p - 1, pr = p + 1;
+        int pl = pc - 1, pr = pc + 1;
         bool hasl = true, hasr = true;
-        if (pl < 0) { if (a.periodic) pl += P; else hasl = false; }
-        if (pr >= P) { if (a.periodic) pr -= P; else hasr = false; }
+        if (pl < 0) { if (!BND) pl += P; else hasl = false; }
+        if (pr >= P) { if (!BND) pr -= P; else hasr = false; }
 #pragma unroll
         for (int t = M - 1; t >= 0; t--) w[t + 4] = w[t];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            w[k] = (exch && hasl) ? halo[(8 * pl + 4 + k) * 64 + lane] : T(0);
-            w[M + 4 + k] = (exch && hasr) ? halo[(8 * pr + k) * 64 + lane] : T(0);
+            w[k] = (exch && hasl) ? halo[(8 * pl + 4 + k) * NPEN + pen] : T(0);
+            w[M + 4 + k] = (exch && hasr) ? halo[(8 * pr + k) * NPEN + pen] : T(0);
         }
-        if (edge_left) {
-            const T *bl = tab + a.off_bl;
-#pragma unroll
-            for (int r = 0; r < kBRowsDev; r++) {
-                T s = T(0);
-                if (r < a.nbl) {
-#pragma unroll
-                    for (int c = 0; c < kBColsDev; c++) s += bl[r * kBColsDev + c] * w[4 + c];
-                }
-                lb[r] = s;
-            }
-        }
-        if (edge_right) {
-            const T *br = tab + a.off_br;
-#pragma unroll
-            for (int r = 0; r < kBRowsDev; r++) {
-                T s = T(0);
-                if (r < a.nbr) {
-#pragma unroll
-                    for (int c = 0; c < kBColsDev; c++) s += br[r * kBColsDev + c] * w[M + 3 - c];
-                }
-                rb[r] = s;
-            }
-        }
-        T cst[9];
-#pragma unroll
-        for (int k = 0; k < 9; k++) cst[k] = tab[k];
-#pragma unroll
-        for (int t = 0; t < M; t++) {
-            T s = cst[0] * w[t];
-#pragma unroll
-            for (int k = 1; k < 9; k++) s += cst[k] * w[t + k];
-            w[t] = s;
-        }
-        if (edge_left) {
-#pragma unroll
-            for (int r = 0; r < kBRowsDev; r++)
-                if (r < a.nbl) w[r] = lb[r];
-        }
-        if (edge_right) {
-#pragma unroll
-            for (int r = 0; r < kBRowsDev; r++)
-                if (r < a.nbr) w[M - 1 - r] = rb[r];
-        }
+        apply_stencil<T, M>(w, tab, bnd, edge_left, edge_right, a.nbl, a.nbr);
     }
 
     // ---------------------------------------------------------------- store
     if constexpr (MODE == 0) {
+        const unsigned hoff = loff + (unsigned)((long long)half * M * stp * (long long)sizeof(T));
         if (valid) {
 #pragma unroll
-            for (int t = 0; t < M; t++) a.out[base + (long long)(pM + t) * st] = w[t];
+            for (int t = 0; t < M; t++) st(ob + (long long)(pw0 + t) * stp, hoff, w[t]);
         }
     } else {
+        constexpr int XT = XTile<PW>::XT, PAD = XTile<PW>::PAD;
+        constexpr int G = 64 / XT;
+        const int myrow = pen * PW + half;
 #pragma unroll
-        for (int c = 0; c < M / kXTile; c++) {
+        for (int c = 0; c < M / XT; c++) {
 #pragma unroll
-            for (int j = 0; j < kXTile; j++) xt[lane * kXPad + j] = w[c * kXTile + j];
+            for (int j = 0; j < XT; j++) xt[myrow * PAD + j] = w[c * XT + j];
             wave_lds_sync();
 #pragma unroll
-            for (int s = 0; s < kXTile; s++) {
-                const int pen = 4 * s + (lane >> 4);
-                const int pt = lane & 15;
-                const long long qq = q0 + pen;
-                if (qq < a.npencils) a.out[qq * n + pM + c * kXTile + pt] = xt[pen * kXPad + pt];
+            for (int s = 0; s < XT; s++) w[c * XT + s] = xt[(s * G + lane / XT) * PAD + (lane % XT)];
+            wave_lds_sync();
+        }
+#pragma unroll
+        for (int c = 0; c < M / XT; c++) {
+#pragma unroll
+            for (int s = 0; s < XT; s++) {
+                const int r = s * G + lane / XT;
+                const int rp = r / PW, rh = r % PW;
+                const long long qq = q0 + rp;
+                if (qq < a.npencils && wave * PW + rh < P) {
+                    const unsigned xo = (unsigned)((rp * (long long)n + rh * M + (lane % XT)) * (long long)sizeof(T));
+                    st(ob + pw0 + c * XT, xo, w[c * XT + s]);
+                }
             }
-            wave_lds_sync();
         }
     }
 }
 
 // ------------------------------------------------------------------------
-// Generic path: one lane per pencil, whole pencil as one partition, the
-// intermediate values kept in the output array (adjoint: in `work`).
+// Generic path: one lane per pencil, whole pencil as one partition (P = 1),
+// the intermediate values kept in the output array (adjoint: in `work`).
 template <typename T, bool ADJ>
 __global__ __launch_bounds__(256) void generic_apply(ApplyArgs<T> a, T *__restrict__ work)
 {
@@ -309,23 +428,24 @@ __global__ __launch_bounds__(256) void generic_apply(ApplyArgs<T> a, T *__restri
     const long long st = a.stride;
     const int n = a.n;
     const T *__restrict__ tab = a.tab;
-    const T *__restrict__ lu = tab + a.off_lu;
-    const T *__restrict__ S = tab + a.off_S;
-    const T *__restrict__ Q = tab + a.off_Q;
+    const T *__restrict__ lu = tab + kOffCoef;          // [n][6]
+    const T *__restrict__ S = lu + 6 * n;               // [n][4]
+    const T *__restrict__ Q = lu + 10 * n;              // [4][4]
     T *y = ADJ ? work : a.out;
 
     auto stencil_row = [&](const T *src, int i) -> T {
         if (!a.periodic) {
             if (i < a.nbl) {
                 T s = T(0);
-                for (int c = 0; c < kBColsDev && c < n; c++) s += tab[a.off_bl + i * kBColsDev + c] * src[base + (long long)c * st];
+                for (int c = 0; c < kBColsDev && c < n; c++)
+                    s += tab[kOffBl + i * kBColsDev + c] * src[base + (long long)c * st];
                 return s;
             }
             if (i >= n - a.nbr) {
                 const int r = n - 1 - i;
                 T s = T(0);
                 for (int c = 0; c < kBColsDev && c < n; c++)
-                    s += tab[a.off_br + r * kBColsDev + c] * src[base + (long long)(n - 1 - c) * st];
+                    s += tab[kOffBr + r * kBColsDev + c] * src[base + (long long)(n - 1 - c) * st];
                 return s;
             }
         }
@@ -344,18 +464,16 @@ __global__ __launch_bounds__(256) void generic_apply(ApplyArgs<T> a, T *__restri
     } else {
         for (int i = 0; i < n; i++) y[base + (long long)i * st] = a.f[base + (long long)i * st];
     }
-    // forward sweep
     T zm1 = y[base], zm2 = T(0);
     for (int i = 1; i < n; i++) {
-        T z = y[base + (long long)i * st] - lu[i * 5 + 0] * zm1 - lu[i * 5 + 1] * zm2;
+        T z = y[base + (long long)i * st] - lu[i * 6 + 0] * zm1 - lu[i * 6 + 1] * zm2;
         y[base + (long long)i * st] = z;
         zm2 = zm1;
         zm1 = z;
     }
-    // backward sweep
     T gp1 = T(0), gp2 = T(0);
     for (int i = n - 1; i >= 0; i--) {
-        T g = y[base + (long long)i * st] * lu[i * 5 + 2] - lu[i * 5 + 3] * gp1 - lu[i * 5 + 4] * gp2;
+        T g = y[base + (long long)i * st] * lu[i * 6 + 2] - lu[i * 6 + 3] * gp1 - lu[i * 6 + 4] * gp2;
         y[base + (long long)i * st] = g;
         gp2 = gp1;
         gp1 = g;
@@ -378,38 +496,51 @@ __global__ __launch_bounds__(256) void generic_apply(ApplyArgs<T> a, T *__restri
 // ------------------------------------------------------------------------
 // launchers
 
-template <typename T, int M, bool ADJ, int MODE>
+template <typename T, int M, int PW, bool ADJ, int MODE>
 static hipError_t launch_spike_t(const ApplyArgs<T> &a, hipStream_t s)
 {
-    const long long nblk = (a.npencils + 63) / 64;
-    const size_t lds = spike_lds_bytes(sizeof(T), a.P, ADJ, MODE);
-    hipLaunchKernelGGL((spike_apply<T, M, ADJ, MODE>), dim3((unsigned)nblk), dim3(64 * a.P), lds, s, a);
+    constexpr int NPEN = 64 / PW;
+    const long long nblk = (a.npencils + NPEN - 1) / NPEN;
+    const int waves = (a.P + PW - 1) / PW;
+    const size_t lds = spike_lds_bytes(sizeof(T), M, a.P, PW, ADJ, MODE, a.coef_len);
+    if (a.periodic)
+        hipLaunchKernelGGL((spike_apply<T, M, PW, ADJ, MODE, false>), dim3((unsigned)nblk), dim3(64 * waves), lds,
+                           s, a);
+    else
+        hipLaunchKernelGGL((spike_apply<T, M, PW, ADJ, MODE, true>), dim3((unsigned)nblk), dim3(64 * waves), lds,
+                           s, a);
     return hipGetLastError();
 }
 
-template <typename T, int M>
-static hipError_t launch_spike_m(const ApplyArgs<T> &a, bool adj, int mode, hipStream_t s)
+template <typename T, int M, int PW>
+static hipError_t launch_spike_mp(const ApplyArgs<T> &a, bool adj, int mode, hipStream_t s)
 {
-    if (!adj) return mode == 0 ? launch_spike_t<T, M, false, 0>(a, s) : launch_spike_t<T, M, false, 1>(a, s);
-    return mode == 0 ? launch_spike_t<T, M, true, 0>(a, s) : launch_spike_t<T, M, true, 1>(a, s);
-}
-
-size_t spike_lds_bytes(size_t tsize, int P, bool adj, int mode)
-{
-    size_t elems = (size_t)(adj ? 12 : 4) * P * 64;
-    if (mode == 1) elems += (size_t)P * 64 * kXPad;
-    return elems * tsize;
+    if (!adj)
+        return mode == 0 ? launch_spike_t<T, M, PW, false, 0>(a, s) : launch_spike_t<T, M, PW, false, 1>(a, s);
+    return mode == 0 ? launch_spike_t<T, M, PW, true, 0>(a, s) : launch_spike_t<T, M, PW, true, 1>(a, s);
 }
 
 template <typename T>
-hipError_t launch_spike(const ApplyArgs<T> &a, int M, bool adj, int mode, hipStream_t s)
+hipError_t launch_spike(const ApplyArgs<T> &a, int M, int PW, bool adj, int mode, hipStream_t s)
 {
-    switch (M) {
-    case 16: return launch_spike_m<T, 16>(a, adj, mode, s);
-    case 32: return launch_spike_m<T, 32>(a, adj, mode, s);
-    case 64: return launch_spike_m<T, 64>(a, adj, mode, s);
-    default: return hipErrorInvalidValue;
+    if (a.P < 1 || a.P > kMaxPartitions || (a.P + PW - 1) / PW > 8) return hipErrorInvalidValue;
+    if (PW == 1) {
+        switch (M) {
+        case 16: return launch_spike_mp<T, 16, 1>(a, adj, mode, s);
+        case 32: return launch_spike_mp<T, 32, 1>(a, adj, mode, s);
+        case 64: return launch_spike_mp<T, 64, 1>(a, adj, mode, s);
+        default: return hipErrorInvalidValue;
+        }
     }
+    if (PW == 2) {
+        switch (M) {
+        case 16: return launch_spike_mp<T, 16, 2>(a, adj, mode, s);
+        case 32: return launch_spike_mp<T, 32, 2>(a, adj, mode, s);
+        case 64: return launch_spike_mp<T, 64, 2>(a, adj, mode, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    return hipErrorInvalidValue;
 }
 
 template <typename T>
@@ -424,8 +555,8 @@ hipError_t launch_generic(const ApplyArgs<T> &a, bool adj, T *work, hipStream_t 
     return hipGetLastError();
 }
 
-template hipError_t launch_spike<double>(const ApplyArgs<double> &, int, bool, int, hipStream_t);
-template hipError_t launch_spike<float>(const ApplyArgs<float> &, int, bool, int, hipStream_t);
+template hipError_t launch_spike<double>(const ApplyArgs<double> &, int, int, bool, int, hipStream_t);
+template hipError_t launch_spike<float>(const ApplyArgs<float> &, int, int, bool, int, hipStream_t);
 template hipError_t launch_generic<double>(const ApplyArgs<double> &, bool, double *, hipStream_t);
 template hipError_t launch_generic<float>(const ApplyArgs<float> &, bool, float *, hipStream_t);
 

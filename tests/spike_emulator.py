@@ -16,14 +16,17 @@ OPER = {"d1": 1, "d2": 2, "filter": 3}
 
 def export(kind, which, n, dx, periodic, bc1, bcn, adjoint, M=0):
     lib = _lib.load()
-    meta = (ctypes.c_int * 12)()
+    meta = (ctypes.c_int * 32)()
     _lib.check(lib.pade_plan_export(KIND[kind], OPER[which], n, dx, int(periodic), bc1, bcn, int(adjoint), M,
                                     None, 0, meta))
-    tab = np.zeros(meta[11])
+    tab = np.zeros(meta[12])
     _lib.check(lib.pade_plan_export(KIND[kind], OPER[which], n, dx, int(periodic), bc1, bcn, int(adjoint), M,
                                     tab.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), tab.size, meta))
-    keys = ["n", "M", "P", "periodic", "nbl", "nbr", "off_bl", "off_br", "off_lu", "off_S", "off_Q", "len"]
-    return tab, dict(zip(keys, list(meta)))
+    keys = ["n", "M", "P", "periodic", "nbl", "nbr", "off_bl", "off_br", "off_coef", "coef_len", "ncls", "kq",
+            "len"]
+    m = dict(zip(keys, list(meta)[:13]))
+    m["cls"] = list(meta)[16:32]
+    return tab, m
 
 
 def _stencil(tab, m, v, pad_mode):
@@ -49,9 +52,20 @@ def _stencil(tab, m, v, pad_mode):
 
 def _solve(tab, m, r):
     n, M, P = m["n"], m["M"], m["P"]
-    lu = tab[m["off_lu"]:m["off_lu"] + P * M * 5].reshape(P, M, 5)
-    S = tab[m["off_S"]:m["off_S"] + P * M * 4].reshape(P, M, 4)
-    Q = tab[m["off_Q"]:m["off_Q"] + P * 4 * 4 * P].reshape(P, 4, 4 * P)
+    ca = tab[m["off_coef"]:m["off_coef"] + m["coef_len"]]
+    ncls = m["ncls"]
+    blocks = ca[:ncls * 10 * M].reshape(ncls, 10 * M)
+    cls = [m["cls"][p] if not m["periodic"] else 0 for p in range(P)]
+    lu = np.stack([blocks[cls[p], :6 * M].reshape(M, 6) for p in range(P)])
+    S = np.stack([blocks[cls[p], 6 * M:].reshape(M, 4) for p in range(P)])
+    qa = ca[ncls * 10 * M:]
+    if m["periodic"]:
+        Q0 = qa[:16 * P].reshape(4, 4 * P)
+        # partition p's rows are partition 0's rotated by p partitions
+        Q = np.stack([np.roll(Q0.reshape(4, P, 4), p, axis=1).reshape(4, 4 * P) for p in range(P)])
+    else:
+        Q = qa[:P * 16 * P].reshape(P, 4, 4 * P)
+    kq = m["kq"]
     g = r.reshape(P, M).copy()
     for p in range(P):
         w = g[p]
@@ -64,7 +78,15 @@ def _solve(tab, m, r):
     if m["periodic"] or P > 1:
         G = np.stack([g[:, 0], g[:, 1], g[:, M - 2], g[:, M - 1]], axis=1).reshape(-1)
         for p in range(P):
-            u = Q[p] @ G
+            # only partitions within the window kq contribute (as on the device)
+            mask = np.zeros(4 * P)
+            for qq in range(P):
+                d = abs(qq - p)
+                if m["periodic"]:
+                    d = min(d, P - d)
+                if 2 * kq + 1 >= P or d <= kq:
+                    mask[4 * qq:4 * qq + 4] = 1.0
+            u = Q[p] @ (G * mask)
             g[p] -= S[p] @ u
     return g.reshape(-1)
 
@@ -73,7 +95,7 @@ def apply_pencil(kind, which, v, dx, periodic, bc1=0, bcn=0, adjoint=False, M=0)
     n = v.size
     tab, m = export(kind, which, n, dx, periodic, bc1, bcn, adjoint, M)
     pad = "wrap" if periodic else "clamp"
-    if not adjoint:
+    if not adjoint or periodic:   # periodic adjoint: transposed tables, forward order
         return _solve(tab, m, _stencil(tab, m, v, pad))
     u = _solve(tab, m, v)
     return _stencil(tab, m, u, pad)

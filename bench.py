@@ -85,6 +85,18 @@ def cpu_baseline(n, dx, nthreads):
     return res, f"{n}x{na}x{nb} slab of pencils (x-derivative), median of repeated applies"
 
 
+def committed_traffic(n):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings)."""
+    path = os.path.join(ROOT, "profiles", "cd10_ddx_512_fp64.md")
+    if n != 512 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +105,9 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="time only the headline kernel")
+    ap.add_argument("--traffic-driver", action="store_true",
+                    help="profiling driver: 3 plain 1 GiB copies (counter calibration), then cd10 "
+                         "periodic dd1, dd2, dd3 three times each; prints nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build targets one GPU: the operator path is measured on one "
@@ -105,6 +120,20 @@ def main():
     from padeops_amd import fields, spectral, _lib
 
     n = args.n
+    if args.traffic_driver:
+        f = po.fortran_empty((n, n, n), device="cuda:0")
+        f.normal_()
+        g = torch.empty_like(f)
+        for _ in range(3):
+            g.copy_(f)
+        op = po.cd10()
+        assert op.init(n, 2 * math.pi / n, True) == 0
+        for fn in (op.dd1, op.dd2, op.dd3):
+            for _ in range(3):
+                fn(f, g, n, n)
+        torch.cuda.synchronize()
+        print("traffic driver done", flush=True)
+        return
     dx = 2 * math.pi / n
     dev = "cuda:0"
     log(f"bench: generating {n}^3 k^-5/3 field on {dev}")
@@ -181,7 +210,7 @@ def main():
         "config": {"workload": f"CD10 periodic ddx, {n}^3 FP64 (precision-study configuration)",
                    "grid": [n, n, n], "operator": "cd10 d1 periodic", "axis": "x"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": committed_traffic(n)},
         "cpu_baseline": cpu,
         "extra": extra,
     }

@@ -207,3 +207,13 @@ def test_permute_bitwise(perm, dtype):
     exp = {1: xf.transpose(1, 0, 2), 2: xf.transpose(2, 1, 0), 3: xf.transpose(0, 2, 1)}[perm]
     got = y.cpu().numpy().reshape(-1)
     assert np.array_equal(got, np.asfortranarray(exp).reshape(-1, order="F"))
+
+
+def test_golden_fixtures():
+    """Every committed fixture case through the C ABI (FP64, 1e-12)."""
+    import golden_cases
+    for c in golden_cases.cases():
+        op = make(c["kind"], c["n"], c["dx"], c["periodic"], *(c["bc"] if c["kind"] == "cd06" else (0, 0)))
+        bc = None if c["kind"] == "cd06" else c["bc"]
+        out = run(op, c["kind"], c["which"], c["axis"], c["f"], bc=bc)
+        assert relerr(out, c["y"]) < TOL64, (c["name"], relerr(out, c["y"]))

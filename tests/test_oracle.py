@@ -1,0 +1,255 @@
+"""CPU pins of the oracle (oracle/pade_dense.py, oracle/pade_banded.c).
+
+Each check is chosen so that a plausible transcription mistake (a dropped
+term, a wrong sign or index, a misread closure parameter) fails one of them:
+
+* periodic operators vs the closed-form modified wavenumbers / transfer
+  function exactly as the reference writes them (src/utilities/spectral.F90:
+  938-1010, tests/test_cf90.F90:108-116) -- pins every interior coefficient;
+* bounded bc = +-1 rows (transcribed row by row from the reference) vs a
+  generic mirror-extension construction written here independently;
+* bounded bc = 0 closures: exact conservation (column sums of the weighted B
+  telescope) and polynomial exactness through degree 4 -- pins the closure
+  parameters (alpha'', alpha''', w1..w4 of cd10.F90:35-80, cd06.F90:29-59);
+* the reference's own analytic test cases (tests/test_cd10.F90,
+  test_cd10_NP.F90, test_cd06.F90, test_filters_NP.F90, test_operators.F90);
+* the C banded restatement vs the dense LAPACK restatement.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pade_dense as pd
+from oracle import banded
+
+OPS = [("cd06", "d1"), ("cd06", "d2"), ("cd10", "d1"), ("cd10", "d2"), ("cf90", "filter")]
+
+
+# --- closed forms as written in the reference's spectral.F90 -------------------
+def ref_cd10_modwave(k, dx):           # spectral.F90:950-961
+    a, b, c = (17 / 12) / 2, (101 / 150) / 4, (1 / 100) / 6
+    al, be = 1 / 2, 1 / 20
+    w = k * dx
+    return (2 * a * np.sin(w) + 2 * b * np.sin(2 * w) + 2 * c * np.sin(3 * w)) / (
+        1 + 2 * al * np.cos(w) + 2 * be * np.cos(2 * w)) / dx
+
+
+def ref_cd10_d2_modwave(k, dx):        # spectral.F90:965-984
+    alpha, beta = 334 / 899, 43 / 1798
+    a, b, c = 1065 / 1798, (1038 / 899) / 4 * 4, (79 / 1798) / 9 * 9
+    w = k * dx
+    wpp = (2 * a * (1 - np.cos(w)) + (b / 2) * (1 - np.cos(2 * w)) + (2 * c / 9) * (1 - np.cos(3 * w))) / (
+        1 + 2 * alpha * np.cos(w) + 2 * beta * np.cos(2 * w))
+    return wpp / dx ** 2
+
+
+def ref_cd06_modwave(k, dx):           # spectral.F90:987-1010
+    alpha = 1 / 3
+    a = (2 / 9) * (8 - 3 * alpha)
+    b = (1 / 18) * (-17 + 57 * alpha)
+    w = k * dx
+    return (a * np.sin(w) + (b / 2) * np.sin(2 * w)) / (1 + 2 * alpha * np.cos(w)) / dx
+
+
+def ref_cf90_transfer(k, dx):          # spectral.F90:938-948, tests/test_cf90.F90:108-116
+    al, be = 6.6624e-1, 1.6688e-1
+    a, b, c, d, e = 9.9965e-1, 6.6652e-1, 1.6674e-1, 4.0e-5, -5.0e-6
+    w = k * dx
+    return (a + 2 * b * np.cos(w) + 2 * c * np.cos(2 * w) + 2 * d * np.cos(3 * w) + 2 * e * np.cos(4 * w)) / (
+        1 + 2 * al * np.cos(w) + 2 * be * np.cos(2 * w))
+
+
+def cd06_d2_closed_form(k, dx):
+    # Lele's 6th-order tridiagonal second derivative (alpha = 2/11, a = 12/11,
+    # b = 3/11); no reference version exists -- this pins the operator the
+    # library defines, not reference output ("parity unpinned")
+    al, a, b = 2 / 11, 12 / 11, 3 / 11
+    w = k * dx
+    return (2 * a * (1 - np.cos(w)) + (b / 2) * (1 - np.cos(2 * w))) / (1 + 2 * al * np.cos(w)) / dx ** 2
+
+
+@pytest.mark.parametrize("n", [16, 33, 64])
+def test_periodic_eigenvalues_match_reference_closed_forms(n):
+    L = 2 * math.pi
+    dx = L / n
+    x = np.arange(n) * dx
+    for kk in range(0, n // 2 + 1):
+        k = 2 * math.pi * kk / L
+        e = np.exp(1j * k * x)
+        for (kind, which), lam in [(("cd10", "d1"), 1j * ref_cd10_modwave(k, dx)),
+                                   (("cd10", "d2"), -ref_cd10_d2_modwave(k, dx)),
+                                   (("cd06", "d1"), 1j * ref_cd06_modwave(k, dx)),
+                                   (("cd06", "d2"), -cd06_d2_closed_form(k, dx)),
+                                   (("cf90", "filter"), ref_cf90_transfer(k, dx))]:
+            if kind == "cf90" and n < 10:
+                continue
+            D = pd.dense_operator(kind, which, n, dx, True)
+            scale = max(1.0, abs(lam))
+            assert np.abs(D @ e - lam * e).max() / scale < 1e-12, (kind, which, n, kk)
+
+
+def _mirror_rows(kind, which, n, bc_left, bc_right):
+    """Bounded A, B built from the interior scheme on a mirror-extended grid:
+    f(-m) = bc f(m) about node 0 and f(n-1+m) = bc f(n-1-m); the output has
+    parity bc (d2, filter) or -bc (d1).  Only the rows the reference treats
+    specially at a bc = +-1 end (A: 2 rows, B: the stencil half-width) are
+    taken from this construction; the rest is the interior scheme."""
+    Aint, Bint = pd.interior(kind, which)
+    H = max(abs(o) for o in Bint)
+    par = -1 if which == "d1" else 1
+    A = np.zeros((n, n))
+    B = np.zeros((n, n))
+
+    def reflect(j, sgn_l, sgn_r):
+        s = 1.0
+        if j < 0:
+            j, s = -j, sgn_l
+        elif j > n - 1:
+            j, s = 2 * (n - 1) - j, sgn_r
+        return j, s
+
+    for i in range(n):
+        for o, v in Aint.items():
+            j, s = reflect(i + o, bc_left * par, bc_right * par)
+            A[i, j] += v * s
+        for o, v in Bint.items():
+            j, s = reflect(i + o, bc_left, bc_right)
+            B[i, j] += v * s
+    return A, B, H
+
+
+@pytest.mark.parametrize("kind,which", [("cd10", "d1"), ("cd10", "d2"), ("cf90", "filter")])
+@pytest.mark.parametrize("bcl", [1, -1])
+@pytest.mark.parametrize("bcr", [1, -1])
+def test_symmetry_rows_are_mirror_extension(kind, which, bcl, bcr):
+    n = 24
+    A, B = pd.assemble(kind, which, n, 1.0, False, bcl, bcr)
+    Am, Bm, H = _mirror_rows(kind, which, n, bcl, bcr)
+    # rows 0..1 / n-2..n-1 of A and 0..H-1 / n-H..n-1 of B
+    for r in list(range(2)) + list(range(n - 2, n)):
+        assert np.allclose(A[r], Am[r], atol=1e-15), (kind, which, bcl, bcr, "A", r)
+    for r in list(range(H)) + list(range(n - H, n)):
+        assert np.allclose(B[r], Bm[r], atol=1e-15), (kind, which, bcl, bcr, "B", r)
+
+
+@pytest.mark.parametrize("kind", ["cd06", "cd10"])
+def test_one_sided_closures_are_conservative(kind):
+    """The weighted one-sided closures make B telescope: every column sum of B
+    vanishes except at the two end columns (cd10.F90:35-80, cd06.F90:29-59)."""
+    n = 40
+    _, B = pd.assemble(kind, "d1", n, 1.0, False, 0, 0)
+    cs = B.sum(axis=0)
+    assert np.abs(cs[1:-1]).max() < 1e-13
+    assert abs(cs[0] + cs[-1]) < 1e-13 and abs(cs[0]) > 0.5
+
+
+@pytest.mark.parametrize("kind,which,maxdeg", [("cd06", "d1", 4), ("cd10", "d1", 4), ("cd10", "d2", 4),
+                                               ("cf90", "filter", 1)])
+def test_one_sided_closures_polynomial_exactness(kind, which, maxdeg):
+    n = 24
+    x = np.linspace(-1.0, 1.0, n)
+    dx = x[1] - x[0]
+    D = pd.dense_operator(kind, which, n, dx, False, 0, 0)
+    for deg in range(maxdeg + 1):
+        f = x ** deg
+        if which == "d1":
+            ex = deg * x ** (deg - 1) if deg else 0 * x
+        elif which == "d2":
+            ex = deg * (deg - 1) * x ** (deg - 2) if deg > 1 else 0 * x
+        else:
+            ex = f
+        assert np.abs(D @ f - ex).max() < 1e-10 * max(1.0, np.abs(ex).max()), (kind, which, deg)
+
+
+# --- the reference's own analytic test programs ---------------------------------
+def test_reference_test_cd10_periodic_sines():
+    """tests/test_cd10.F90:9-81: 32^3 periodic, f = sin x + sin y + sin z."""
+    n = 32
+    dx = 2 * math.pi / n
+    x = np.arange(n) * dx
+    f = np.sin(x)
+    d1 = pd.dense_operator("cd10", "d1", n, dx, True) @ f
+    d2 = pd.dense_operator("cd10", "d2", n, dx, True) @ f
+    assert np.abs(d1 - np.cos(x)).max() < 1e-9
+    assert np.abs(d2 + np.sin(x)).max() < 1e-9
+
+
+def test_reference_test_cd10_np_symmetric_antisymmetric():
+    """tests/test_cd10_NP.F90:29-81: 32 points on [0, pi/2], f = cos x with
+    bc = (1, -1): even at 0, odd at pi/2."""
+    n = 32
+    x = np.linspace(0, math.pi / 2, n)
+    dx = x[1] - x[0]
+    f = np.cos(x)
+    d1 = pd.dense_operator("cd10", "d1", n, dx, False, 1, -1) @ f
+    d2 = pd.dense_operator("cd10", "d2", n, dx, False, 1, -1) @ f
+    assert np.abs(d1 + np.sin(x)).max() < 1e-8
+    assert np.abs(d2 + np.cos(x)).max() < 1e-7
+
+
+def test_reference_test_cd06_sine():
+    """tests/test_cd06.F90:24-52: periodic cd06 ddx of sin x (one pencil)."""
+    n = 256
+    dx = 2 * math.pi / n
+    x = np.arange(n) * dx
+    d1 = pd.dense_operator("cd06", "d1", n, dx, True) @ np.sin(x)
+    assert np.abs(d1 - np.cos(x)).max() < 1e-9
+
+
+def test_reference_test_filters_np_odd_field():
+    """tests/test_filters_NP.F90:68-131: bc (-1,-1), f = sin on [0, 2pi]:
+    odd at both ends, so the bounded filter equals the periodic transfer
+    function of the mirrored signal."""
+    n = 65
+    x = np.linspace(0, 2 * math.pi, n)
+    dx = x[1] - x[0]
+    f = np.sin(3 * x)
+    f[0] = 0.0
+    f[-1] = 0.0
+    got = pd.dense_operator("cf90", "filter", n, dx, False, -1, -1) @ f
+    assert np.abs(got - ref_cf90_transfer(3.0, dx) * f).max() < 1e-12
+
+
+def test_reference_test_operators_linear_quadratic():
+    """tests/test_operators.F90:52-157: bounded cd10 (bc = 0) derivatives of
+    linear and quadratic fields are exact."""
+    n = 20
+    x = np.linspace(0.0, 1.0, n)
+    dx = x[1] - x[0]
+    D = pd.dense_operator("cd10", "d1", n, dx, False, 0, 0)
+    assert np.abs(D @ (3 * x + 1) - 3).max() < 1e-11
+    assert np.abs(D @ (x * x) - 2 * x).max() < 1e-11
+
+
+# --- the two restatements agree ----------------------------------------------------
+@pytest.mark.parametrize("kind,which", OPS)
+@pytest.mark.parametrize("periodic", [True, False])
+def test_banded_c_matches_dense(kind, which, periodic):
+    if kind == "cd06" and which == "d2" and not periodic:
+        pytest.skip("bounded cd06 d2 is a composition")
+    rng = np.random.default_rng(5)
+    bcs = [(0, 0)] if (periodic or kind == "cd06") else [(0, 0), (1, -1), (-1, 1), (1, 1), (0, -1)]
+    for bc in bcs:
+        for n in (16, 37):
+            f = np.asfortranarray(rng.standard_normal((n, 5, 4)))
+            for axis in range(3):
+                g = np.asfortranarray(np.moveaxis(f, 0, axis))
+                ref = pd.apply(kind, which, g, axis, 0.21, periodic, *bc)
+                out = banded.BandedOp(kind, which, n, 0.21, periodic, *bc).apply(g, axis, nthreads=2)
+                assert np.abs(out - ref).max() / np.abs(ref).max() < 1e-12
+
+
+def test_init_error_codes():
+    assert pd.init_error("cd06", 5, True) == 3
+    assert pd.init_error("cd06", 7, False) == 2
+    assert pd.init_error("cd10", 7, True) == 2
+    assert pd.init_error("cf90", 9, False) == 7
+    assert pd.init_error("cd10", 1, False) == 0
+
+
+def test_oracle_reproduces_golden_fixtures():
+    import golden_cases
+    for c in golden_cases.cases():
+        y = pd.apply(c["kind"], c["which"], c["f"], c["axis"] - 1, c["dx"], c["periodic"], *c["bc"])
+        assert np.abs(y - c["y"]).max() <= 1e-13 * np.abs(c["y"]).max(), c["name"]

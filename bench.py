@@ -172,16 +172,34 @@ def main():
                                 "hbm_frac": 8.0 * pts / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                 "err_max_rms": errors(d32.permute(2, 1, 0), ref_x)}
         del f32, d32
-        for name, prec in (("dense_fp32", _lib.PADE_DENSE_F32), ("dense_bf16x3", _lib.PADE_DENSE_BF16X3)):
+        # MFMA dense forms on the same FP32 storage as the FP32 banded solve
+        F = _lib.PADE_DENSE_FULL
+        for name, prec in (("dense_fp32", _lib.PADE_DENSE_F32), ("dense_bf16x3", _lib.PADE_DENSE_BF16X3),
+                           ("dense_fp32_full", _lib.PADE_DENSE_F32 | F),
+                           ("dense_bf16x3_full", _lib.PADE_DENSE_BF16X3 | F)):
+            log(f"bench: {name}")
             try:
+                f32 = po.fortran_empty((n, n, n), dtype=torch.float32, device=dev)
+                f32.copy_(f)
                 d32 = po.fortran_empty((n, n, n), dtype=torch.float32, device=dev)
-                run = lambda: op.dense_apply(_lib.PADE_D1, 1, f, d32, n, n, prec)  # noqa: E731
+                run = lambda: op.dense_apply(_lib.PADE_D1, 1, f32, d32, n, n, prec)  # noqa: E731
                 run()
                 torch.cuda.synchronize()
-                msd = timed(run, max(5, args.steps // 5), 2)
-                extra[name] = {"gpts": pts / (msd * 1e-3) / 1e9,
-                               "err_max_rms": errors(d32.permute(2, 1, 0), ref_x)}
-                del d32
+                err = errors(d32.permute(2, 1, 0), ref_x)
+                full = bool(prec & F)
+                msd = timed(run, max(3, args.steps // (10 if full else 2)), 2)
+                window, halfwidth = op.dense_info(_lib.PADE_D1, prec)
+                bf = (prec & ~F) == _lib.PADE_DENSE_BF16X3
+                flop_pt = 2.0 * window * (6 if bf else 1)
+                peak = BF16_MFMA_TFLOPS if bf else FP32_MFMA_TFLOPS
+                gpts = pts / (msd * 1e-3) / 1e9
+                extra[name] = {"gpts": gpts, "ms": msd, "err_max_rms": list(err),
+                               "accepted": err[0] <= 1e-6,
+                               "window_points": window, "decay_halfwidth": halfwidth,
+                               "mfma_flop_per_pt": flop_pt,
+                               "mfma_frac": gpts * flop_pt / 1e3 / peak,
+                               "hbm_frac": 8.0 * gpts / HBM_PEAK_GBS}
+                del f32, d32
             except po.PadeError as e:
                 extra[name] = {"unavailable": str(e)}
         del ref_x

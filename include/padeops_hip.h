@@ -36,7 +36,7 @@ extern "C" {
 
 enum pade_kind { PADE_CD06 = 1, PADE_CD10 = 2, PADE_CF90 = 3 };
 enum pade_oper { PADE_D1 = 1, PADE_D2 = 2, PADE_FILTER = 3 };
-enum pade_dense_prec { PADE_DENSE_F32 = 1, PADE_DENSE_BF16X3 = 2 };
+enum pade_dense_prec { PADE_DENSE_F32 = 1, PADE_DENSE_BF16X3 = 2, PADE_DENSE_FULL = 16 };
 enum pade_dtype { PADE_F64 = 0, PADE_F32 = 1 };
 
 typedef struct pade_op_s *pade_op_t;
@@ -83,13 +83,24 @@ int pade_op_apply_adjoint_f32(pade_op_t op, int oper, int axis, const float *f, 
 int pade_op_apply_host(pade_op_t op, int oper, int axis, const double *f, double *df, int na,
                        int nb, int bc1, int bcn);
 
-/* Dense-operator form: df = D f with D = A^{-1} B precomputed in FP64 and
- * applied as an MFMA GEMM over all pencils.  prec PADE_DENSE_F32 (FP32
- * operands, FP32 accumulate) or PADE_DENSE_BF16X3 (3-term split BF16 of
- * both operands, FP32 accumulate).  in/out dtype PADE_F64 or PADE_F32.
+/* Dense-operator form of a periodic operator: df = D f with D = A^{-1} B
+ * precomputed in FP64 (circulant) and applied as an MFMA GEMM over all
+ * pencils.  prec PADE_DENSE_F32 (FP32 operands and accumulation) or
+ * PADE_DENSE_BF16X3 (both operands split into three BF16 limbs, six limb
+ * products, FP32 accumulation).  By default tiles of D beyond the operator's
+ * decay band (entries summing to <= 2^-30 of a row's absolute sum) are
+ * skipped; OR PADE_DENSE_FULL into prec to apply all n x n entries.
+ * in/out dtype PADE_F64 or PADE_F32 (FP64 inputs are rounded to FP32 first).
+ * Needs n a multiple of 64; returns 21 for bounded operators.
  * No reference counterpart (precision study). */
 int pade_op_dense_apply(pade_op_t op, int oper, int axis, const void *f, int f_dtype, void *df,
                         int df_dtype, int na, int nb, int bc1, int bcn, int prec, void *stream);
+
+/* Shape of the dense form that pade_op_dense_apply runs for (oper, prec):
+ * *window = input points per output block that the GEMM visits (16 * slots;
+ * n for PADE_DENSE_FULL), *halfwidth = decay half-width kept.  Builds and
+ * caches the form on first use.  No reference counterpart. */
+int pade_op_dense_info(pade_op_t op, int oper, int prec, int *window, int *halfwidth);
 
 /* Pencil re-orientation on one GPU.  2DECOMP's transpose_x_to_y / y_to_z /
  * z_to_y / y_to_x (2decomp_fft/src/transpose_x_to_y.f90:14-91 and siblings)

@@ -7,5 +7,6 @@ kernels for gfx950 behind the C ABI in ``include/padeops_hip.h``.
 """
 from ._lib import PadeError  # noqa: F401
 from .operators import cd06, cd10, cf90, fortran_empty, as_fortran  # noqa: F401
+from .facades import derivatives, filters  # noqa: F401
 
 __version__ = "0.1.0"

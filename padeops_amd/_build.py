@@ -98,6 +98,19 @@ def build_fortran(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True, cwd=moddir)
     os.replace(tmp, FLIB)
+    # the Fortran shim's test program (run by tests/test_gpu_fortran.py)
+    test_src = os.path.join(ROOT, "tests", "fortran", "test_shim.F90")
+    if os.path.exists(test_src):
+        bindir = os.path.join(ROOT, "tests", "fortran", "bin")
+        os.makedirs(bindir, exist_ok=True)
+        exe = os.path.join(bindir, "test_shim")
+        cmd = [fc, "-O2", "-I", moddir, "-J", moddir, "-o", exe + ".tmp", test_src,
+               "-L", LIBDIR, "-lpadeops_f", "-lpadeops_hip",
+               "-Wl,-rpath,$ORIGIN/../../../padeops_amd/lib", "-lc", "-lpthread", "-lm"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True, cwd=moddir)
+        os.replace(exe + ".tmp", exe)
     return FLIB
 
 

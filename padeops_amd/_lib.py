@@ -13,14 +13,14 @@ LIB_PATH = os.path.join(_HERE, "lib", "libpadeops_hip.so")
 
 PADE_CD06, PADE_CD10, PADE_CF90 = 1, 2, 3
 PADE_D1, PADE_D2, PADE_FILTER = 1, 2, 3
-PADE_DENSE_F32, PADE_DENSE_BF16X3 = 1, 2
+PADE_DENSE_F32, PADE_DENSE_BF16X3, PADE_DENSE_FULL = 1, 2, 16
 PADE_F64, PADE_F32 = 0, 1
 
 # every symbol include/padeops_hip.h declares
 EXPORTS = [
     "pade_op_create", "pade_op_destroy", "pade_op_size", "pade_op_apply", "pade_op_apply_f32",
     "pade_op_apply_adjoint", "pade_op_apply_adjoint_f32", "pade_op_apply_host",
-    "pade_op_dense_apply", "pade_pencil_copy", "pade_permute3", "pade_plan_export",
+    "pade_op_dense_apply", "pade_op_dense_info", "pade_pencil_copy", "pade_permute3", "pade_plan_export",
     "pade_last_error", "pade_version",
 ]
 

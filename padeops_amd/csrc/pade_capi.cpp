@@ -477,24 +477,40 @@ int pade_op_dense_apply(pade_op_t op, int oper, int axis, const void *f, int f_d
         if (!ok(bc1) || !ok(bcn))
             return fail(324, "Incorrect boundary specification (should be 0, 1 or -1)");
     }
-    if (prec != PADE_DENSE_F32 && prec != PADE_DENSE_BF16X3) return fail(400, "unknown dense precision");
+    const bool full = (prec & PADE_DENSE_FULL) != 0;
+    const int pbase = prec & ~PADE_DENSE_FULL;
+    if (pbase != PADE_DENSE_F32 && pbase != PADE_DENSE_BF16X3) return fail(400, "unknown dense precision");
     if ((f_dtype != PADE_F64 && f_dtype != PADE_F32) || (df_dtype != PADE_F64 && df_dtype != PADE_F32))
         return fail(400, "unknown dtype");
-    if (op->n < 16 || op->n % 16 != 0) return fail(21, "dense form needs n a multiple of 16");
-    if (op->kind == pade::CD06 && oper == pade::D2 && !op->periodic)
-        return fail(21, "dense form of the composed bounded cd06 d2 is not provided");
-    const int k1 = op->periodic ? 0 : plan_key_bc(op, bc1);
-    const int kn = op->periodic ? 0 : plan_key_bc(op, bcn);
-    auto it = op->plans.find({oper, k1, kn, 0});
+    if (!op->periodic) return fail(21, "the dense form is provided for periodic operators");
+    if (op->n < 64 || op->n % 64 != 0) return fail(21, "the dense form needs n a multiple of 64");
+    auto it = op->plans.find({oper, 0, 0, 0});
     if (it == op->plans.end()) return fail(21, "no plan for this operation");
-    const pade::DenseMatrix *D = nullptr;
+    const pade::DenseForm *D = nullptr;
     {
         std::lock_guard<std::mutex> lk(op->mu);
-        int rc = pade::dense_get(op->dense, it->second.mats, oper * 100 + (k1 + 1) * 10 + (kn + 1), &D);
+        int rc = pade::dense_get(op->dense, it->second.mats, oper, full, &D);
         if (rc) return fail(rc >= 1000 ? rc : 21, "dense operator construction failed");
     }
-    hipError_t e = pade::dense_launch(*D, axis, f, f_dtype, df, df_dtype, na, nb, prec, (hipStream_t)stream);
+    hipError_t e = pade::dense_launch(*D, axis, f, f_dtype, df, df_dtype, na, nb, pbase, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "dense launch");
+    return 0;
+}
+
+int pade_op_dense_info(pade_op_t op, int oper, int prec, int *window, int *halfwidth)
+{
+    if (!op || !window || !halfwidth) return fail(400, "null argument");
+    if (!oper_valid(op->kind, oper)) return fail(21, "operation not provided for this operator kind");
+    if (!op->periodic) return fail(21, "the dense form is provided for periodic operators");
+    if (op->n < 64 || op->n % 64 != 0) return fail(21, "the dense form needs n a multiple of 64");
+    auto it = op->plans.find({oper, 0, 0, 0});
+    if (it == op->plans.end()) return fail(21, "no plan for this operation");
+    const pade::DenseForm *D = nullptr;
+    std::lock_guard<std::mutex> lk(op->mu);
+    int rc = pade::dense_get(op->dense, it->second.mats, oper, (prec & PADE_DENSE_FULL) != 0, &D);
+    if (rc) return fail(rc >= 1000 ? rc : 21, "dense operator construction failed");
+    *window = 16 * D->S;
+    *halfwidth = D->halfwidth;
     return 0;
 }
 

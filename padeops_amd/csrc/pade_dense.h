@@ -11,22 +11,44 @@
 
 namespace pade {
 
-struct DenseMatrix {
+// Precision codes of the dense form (the public enum pade_dense_prec).
+constexpr int kDenseF32 = 1;
+constexpr int kDenseBF16x3 = 2;
+constexpr int kDenseFull = 16;     // modifier: apply every tile of D (no decay-band skipping)
+
+// A periodic operator is circulant, D(i, j) = d[(j - i) mod n].  The GEMM
+// runs over output blocks of 32 points and input chunks of 16 points; output
+// block b reads the S chunks 2b - c_off .. 2b - c_off + S - 1 (mod n/16), and
+// the 32 x 16 tile of D for window slot s is the same for every b.  Tiles are
+// stored in MFMA operand order:
+//   BF16x3: [S][3 limbs][64 lanes][8]  bf16   (3 KiB per slot)
+//   F32:    [S][64 lanes][8]           float  (2 KiB per slot)
+// with lane l holding D(b*32 + l%32, chunk*16 + 8*(l/32) + e), e = 0..7.
+struct DenseForm {
     int n = 0;
-    float *d32 = nullptr;       // D, FP32, row-major n x n
-    void *dbf = nullptr;        // D as three BF16 limbs, [3][n][n]
+    int S = 0;           // window slots (multiple of kDenseGroup)
+    int c_off = 0;       // first chunk of the window relative to 2b
+    int halfwidth = 0;   // decay half-width kept (n/2 for the full form)
+    void *tiles_bf = nullptr;
+    float *tiles_f32 = nullptr;
 };
+
+constexpr int kDenseGroup = 4;       // chunks loaded per pipeline step
 
 struct DenseCache {
-    std::map<int, DenseMatrix> mats;
+    std::map<long long, DenseForm> forms;
 };
 
-// Build (once) and return the dense operator for a plan key.
-int dense_get(DenseCache &cache, const OpMatrices &m, int key, const DenseMatrix **out);
+// Build (once) and return the dense form of a periodic operator.  `key`
+// identifies the operator within its handle.  Returns 21 when the dense form
+// is not provided for this operator (bounded, or n not a multiple of 32).
+int dense_get(DenseCache &cache, const OpMatrices &m, int key, bool full, const DenseForm **out);
 void dense_release(DenseCache &cache);
 
-hipError_t dense_launch(const DenseMatrix &D, int axis, const void *f, int f_dtype, void *df,
-                        int df_dtype, int na, int nb, int prec, hipStream_t s);
+// out = D f along `axis` (1: f(n,na,nb), 2: f(na,n,nb), 3: f(na,nb,n)).
+// f_dtype / df_dtype: 0 FP64, 1 FP32.  prec: kDenseF32 or kDenseBF16x3.
+hipError_t dense_launch(const DenseForm &D, int axis, const void *f, int f_dtype, void *df, int df_dtype,
+                        int na, int nb, int prec, hipStream_t s);
 
 hipError_t permute_launch(const void *src, void *dst, int elem_size, int n1, int n2, int n3, int perm,
                           hipStream_t s);

@@ -4,72 +4,560 @@
 #include <cstdint>
 #include <cstring>
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "pade_dense.h"
 
 namespace pade {
 
-int dense_get(DenseCache &cache, const OpMatrices &m, int key, const DenseMatrix **out)
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+uint16_t bf16_rne(float x)  // round to nearest even (finite inputs)
 {
-    auto it = cache.mats.find(key);
-    if (it != cache.mats.end()) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+
+double bf16_value(uint16_t h)
+{
+    uint32_t u = uint32_t(h) << 16;
+    float x;
+    std::memcpy(&x, &u, 4);
+    return double(x);
+}
+
+long long wrap_mod(long long a, long long n) { return ((a % n) + n) % n; }
+
+}  // namespace
+
+int dense_get(DenseCache &cache, const OpMatrices &m, int key, bool full, const DenseForm **out)
+{
+    const long long ck = 2LL * key + (full ? 1 : 0);
+    auto it = cache.forms.find(ck);
+    if (it != cache.forms.end()) {
         *out = &it->second;
         return 0;
     }
+    const int n = m.n;
+    if (!m.periodic || n < 64 || n % 64 != 0) return 21;
     std::vector<double> D;
     if (int rc = dense_operator(m, D)) return rc;
-    DenseMatrix dm;
-    dm.n = m.n;
-    const size_t nn = (size_t)m.n * m.n;
-    std::vector<float> d32(nn);
-    std::vector<uint16_t> bf(3 * nn);
-    auto to_bf16 = [](float x) -> uint16_t {  // round to nearest even
-        uint32_t u;
-        std::memcpy(&u, &x, 4);
-        const uint32_t lsb = (u >> 16) & 1u;
-        u += 0x7fffu + lsb;
-        return uint16_t(u >> 16);
-    };
-    auto from_bf16 = [](uint16_t h) -> double {
-        uint32_t u = uint32_t(h) << 16;
-        float x;
-        std::memcpy(&x, &u, 4);
-        return double(x);
-    };
-    for (size_t k = 0; k < nn; k++) {
-        d32[k] = float(D[k]);
-        double r = D[k];
-        for (int l = 0; l < 3; l++) {
-            const uint16_t h = to_bf16(float(r));
-            bf[l * nn + k] = h;
-            r -= from_bf16(h);
+    // circulant row: D(0, j) = d[j]
+    std::vector<double> d(D.begin(), D.begin() + n);
+    const int nc = n / 16;
+    DenseForm F;
+    F.n = n;
+    // Decay band: the smallest half-width W whose dropped entries sum to at
+    // most 2^-30 of the row's absolute sum (far below FP32 resolution).
+    double total = 0.0;
+    for (double v : d) total += std::fabs(v);
+    int W = n / 2;
+    for (int w = 0; w < n / 2; w++) {
+        double tail = 0.0;
+        for (int j = 0; j < n; j++) {
+            const int dist = std::min(j, n - j);
+            if (dist > w) tail += std::fabs(d[j]);
+        }
+        if (tail <= std::ldexp(total, -30)) {
+            W = w;
+            break;
         }
     }
-    hipError_t e = hipMalloc(&dm.d32, nn * sizeof(float));
+    const int cw = (W + 15) / 16;
+    int S = 2 + 2 * cw;
+    S = (S + kDenseGroup - 1) / kDenseGroup * kDenseGroup;
+    if (full || S >= nc) {
+        F.S = nc;
+        F.c_off = 0;
+        F.halfwidth = n / 2;
+    } else {
+        F.S = S;
+        F.c_off = cw;
+        F.halfwidth = W;
+    }
+    const size_t nbf = (size_t)F.S * 3 * 64 * 8, nf = (size_t)F.S * 2 * 64 * 4;
+    std::vector<uint16_t> tb(nbf);
+    std::vector<float> tf(nf);
+    for (int s = 0; s < F.S; s++)
+        for (int lane = 0; lane < 64; lane++) {
+            const int r = lane % 32, g = lane / 32;
+            for (int e = 0; e < 8; e++) {
+                const int kk = 8 * g + e;
+                const double v = d[wrap_mod(16LL * (s - F.c_off) + kk - r, n)];
+                double rem = v;
+                for (int L = 0; L < 3; L++) {
+                    const uint16_t h = bf16_rne(float(rem));
+                    tb[(((size_t)s * 3 + L) * 64 + lane) * 8 + e] = h;
+                    rem -= bf16_value(h);
+                }
+                tf[(((size_t)s * 2 + e / 4) * 64 + lane) * 4 + e % 4] = float(v);
+            }
+        }
+    hipError_t e = hipMalloc(&F.tiles_bf, nbf * sizeof(uint16_t));
     if (e != hipSuccess) return 1000 + int(e);
-    e = hipMalloc(&dm.dbf, 3 * nn * sizeof(uint16_t));
-    if (e != hipSuccess) { hipFree(dm.d32); return 1000 + int(e); }
-    hipMemcpy(dm.d32, d32.data(), nn * sizeof(float), hipMemcpyHostToDevice);
-    hipMemcpy(dm.dbf, bf.data(), 3 * nn * sizeof(uint16_t), hipMemcpyHostToDevice);
-    auto res = cache.mats.emplace(key, dm);
+    e = hipMalloc(&F.tiles_f32, nf * sizeof(float));
+    if (e != hipSuccess) {
+        (void)hipFree(F.tiles_bf);
+        return 1000 + int(e);
+    }
+    e = hipMemcpy(F.tiles_bf, tb.data(), nbf * sizeof(uint16_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(F.tiles_f32, tf.data(), nf * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(F.tiles_bf);
+        (void)hipFree(F.tiles_f32);
+        return 1000 + int(e);
+    }
+    auto res = cache.forms.emplace(ck, F);
     *out = &res.first->second;
     return 0;
 }
 
 void dense_release(DenseCache &cache)
 {
-    for (auto &kv : cache.mats) {
-        if (kv.second.d32) hipFree(kv.second.d32);
-        if (kv.second.dbf) hipFree(kv.second.dbf);
+    for (auto &kv : cache.forms) {
+        if (kv.second.tiles_bf) (void)hipFree(kv.second.tiles_bf);
+        if (kv.second.tiles_f32) (void)hipFree(kv.second.tiles_f32);
     }
-    cache.mats.clear();
+    cache.forms.clear();
 }
 
-hipError_t dense_launch(const DenseMatrix &, int, const void *, int, void *, int, int, int, int,
-                        hipStream_t)
+// ------------------------------------------------------------------------
+// Dense GEMM over pencils.  Output rows are the 32 points of an output block,
+// columns the 32 pencils of a pencil tile, K runs over the window's input
+// chunks.  MFMA operand order (gfx950, 32x32 tiles): the A operand lane l
+// holds row l%32, the B operand lane l holds column l%32, and both hold the
+// same 8 K positions 8*(l/32)+e, so the product pairs them consistently; the
+// 16 accumulator values of lane l are column l%32, rows
+// 8*(r/4) + 4*(l/32) + r%4.  Each wave owns NPT pencil tiles and sweeps all
+// output blocks, loading the next group of chunks while the MFMAs of the
+// current group run.  BF16x3 splits every input value into three BF16 limbs
+// (round to nearest even on the running remainder) and forms the six
+// products whose limb orders sum to at most 2; the leading product
+// accumulates apart from the five correction products.
+struct DenseArgs {
+    const void *f;
+    void *out;
+    const void *tiles;
+    long long npencils, A1, B1, B2, stride;
+    int n, S, c_off;
+};
+
+__device__ __forceinline__ unsigned bf16_bits(float x)  // x rounded to BF16, as FP32 bits
 {
-    return hipErrorNotSupported;
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u & 0xffff0000u;
+}
+
+__device__ __forceinline__ void split3(const float (&x)[8], u32x4 &l0, u32x4 &l1, u32x4 &l2)
+{
+    unsigned h0[8], h1[8], h2[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        float r = x[e];
+        h0[e] = bf16_bits(r);
+        r -= __uint_as_float(h0[e]);
+        h1[e] = bf16_bits(r);
+        r -= __uint_as_float(h1[e]);
+        h2[e] = bf16_bits(r);
+    }
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        l0[w] = (h0[2 * w] >> 16) | h0[2 * w + 1];
+        l1[w] = (h1[2 * w] >> 16) | h1[2 * w + 1];
+        l2[w] = (h2[2 * w] >> 16) | h2[2 * w + 1];
+    }
+}
+
+__device__ __forceinline__ f32x16 mfma_bf16(u32x4 a, u32x4 b, f32x16 c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                   c, 0, 0, 0);
+}
+
+template <typename TI, bool CONTIG>
+__device__ __forceinline__ void load8(const TI *__restrict__ f, long long off, long long stride, TI (&x)[8])
+{
+    if constexpr (CONTIG) {
+        if constexpr (sizeof(TI) == 4) {
+            const float4 *p = reinterpret_cast<const float4 *>(f + off);
+            const float4 a = p[0], b = p[1];
+            x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
+            x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
+        } else {
+            const double2 *p = reinterpret_cast<const double2 *>(f + off);
+#pragma unroll
+            for (int h = 0; h < 4; h++) {
+                const double2 v = p[h];
+                x[2 * h] = v.x;
+                x[2 * h + 1] = v.y;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; e++) x[e] = f[off + e * stride];
+    }
+}
+
+template <typename TO, bool CONTIG>
+__device__ __forceinline__ void store4(TO *__restrict__ o, long long off, long long stride, float v0, float v1,
+                                       float v2, float v3)
+{
+    if constexpr (CONTIG) {
+        if constexpr (sizeof(TO) == 4) {
+            *reinterpret_cast<float4 *>(o + off) = make_float4(v0, v1, v2, v3);
+        } else {
+            double2 *p = reinterpret_cast<double2 *>(o + off);
+            p[0] = make_double2(v0, v1);
+            p[1] = make_double2(v2, v3);
+        }
+    } else {
+        o[off] = TO(v0);
+        o[off + stride] = TO(v1);
+        o[off + 2 * stride] = TO(v2);
+        o[off + 3 * stride] = TO(v3);
+    }
+}
+
+template <int PREC, typename TI, typename TO, bool CONTIG, int NPT>
+__global__ __launch_bounds__(256) void dense_apply(DenseArgs a)
+{
+    constexpr int G = kDenseGroup;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    u32x4 *tl = reinterpret_cast<u32x4 *>(smem_raw);
+    const int tile_u4 = PREC == kDenseBF16x3 ? 3 * 64 : 2 * 64;   // 16-byte words per window slot
+    {
+        const u32x4 *src = reinterpret_cast<const u32x4 *>(a.tiles);
+        for (int i = threadIdx.x; i < a.S * tile_u4; i += 256) tl[i] = src[i];
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int pl = lane & 31, g = lane >> 5;
+    const long long q0 = ((long long)blockIdx.x * 4 + wave) * (32 * NPT);
+    if (q0 >= a.npencils) return;       // no barrier follows
+    const TI *__restrict__ f = reinterpret_cast<const TI *>(a.f);
+    TO *__restrict__ o = reinterpret_cast<TO *>(a.out);
+    long long base[NPT];
+    bool valid[NPT];
+#pragma unroll
+    for (int t = 0; t < NPT; t++) {
+        const long long q = q0 + 32 * t + pl;
+        valid[t] = q < a.npencils;
+        const long long qc = valid[t] ? q : a.npencils - 1;
+        base[t] = (qc % a.A1) * a.B1 + (qc / a.A1) * a.B2;
+    }
+    const int nc = a.n / 16, nblk = a.n / 32;
+    const int ngrp = a.S / G, nsteps = nblk * ngrp;
+
+    TI raw[NPT][G][8];
+    auto issue = [&](int u) {
+        const int ib = u / ngrp, grp = u - ib * ngrp;
+        int c = 2 * ib - a.c_off + grp * G + nc;
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+            const int cc = c + j >= nc ? (c + j >= 2 * nc ? c + j - 2 * nc : c + j - nc) : c + j;
+            const long long k0 = 16LL * cc + 8 * g;
+#pragma unroll
+            for (int t = 0; t < NPT; t++) load8<TI, CONTIG>(f, base[t] + k0 * a.stride, a.stride, raw[t][j]);
+        }
+    };
+    issue(0);
+    f32x16 hi[NPT], lo[NPT];
+#pragma unroll
+    for (int t = 0; t < NPT; t++) {
+        hi[t] = f32x16{};
+        lo[t] = f32x16{};
+    }
+    for (int u = 0; u < nsteps; u++) {
+        float x[NPT][G][8];
+#pragma unroll
+        for (int t = 0; t < NPT; t++)
+#pragma unroll
+            for (int j = 0; j < G; j++)
+#pragma unroll
+                for (int e = 0; e < 8; e++) x[t][j][e] = float(raw[t][j][e]);
+        if (u + 1 < nsteps) issue(u + 1);
+        const int ib = u / ngrp, grp = u - ib * ngrp;
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+            const int s = grp * G + j;
+            if constexpr (PREC == kDenseBF16x3) {
+                const u32x4 a0 = tl[(s * 3 + 0) * 64 + lane];
+                const u32x4 a1 = tl[(s * 3 + 1) * 64 + lane];
+                const u32x4 a2 = tl[(s * 3 + 2) * 64 + lane];
+#pragma unroll
+                for (int t = 0; t < NPT; t++) {
+                    u32x4 b0, b1, b2;
+                    split3(x[t][j], b0, b1, b2);
+                    hi[t] = mfma_bf16(a0, b0, hi[t]);
+                    lo[t] = mfma_bf16(a0, b1, lo[t]);
+                    lo[t] = mfma_bf16(a1, b0, lo[t]);
+                    lo[t] = mfma_bf16(a0, b2, lo[t]);
+                    lo[t] = mfma_bf16(a1, b1, lo[t]);
+                    lo[t] = mfma_bf16(a2, b0, lo[t]);
+                }
+            } else {
+                const u32x4 a0 = tl[(s * 2 + 0) * 64 + lane];
+                const u32x4 a1 = tl[(s * 2 + 1) * 64 + lane];
+                const float av[8] = {__uint_as_float(a0[0]), __uint_as_float(a0[1]), __uint_as_float(a0[2]),
+                                     __uint_as_float(a0[3]), __uint_as_float(a1[0]), __uint_as_float(a1[1]),
+                                     __uint_as_float(a1[2]), __uint_as_float(a1[3])};
+#pragma unroll
+                for (int t = 0; t < NPT; t++)
+#pragma unroll
+                    for (int e = 0; e < 8; e += 2) {
+                        hi[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e], x[t][j][e], hi[t], 0, 0, 0);
+                        lo[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e + 1], x[t][j][e + 1], lo[t], 0, 0, 0);
+                    }
+            }
+        }
+        if (grp == ngrp - 1) {
+#pragma unroll
+            for (int t = 0; t < NPT; t++) {
+                if (valid[t]) {
+#pragma unroll
+                    for (int qd = 0; qd < 4; qd++) {
+                        const long long row = 32LL * ib + 8 * qd + 4 * g;
+                        store4<TO, CONTIG>(o, base[t] + row * a.stride, a.stride,
+                                           hi[t][4 * qd] + lo[t][4 * qd], hi[t][4 * qd + 1] + lo[t][4 * qd + 1],
+                                           hi[t][4 * qd + 2] + lo[t][4 * qd + 2],
+                                           hi[t][4 * qd + 3] + lo[t][4 * qd + 3]);
+                    }
+                }
+                hi[t] = f32x16{};
+                lo[t] = f32x16{};
+            }
+        }
+    }
+}
+
+// Register-ring variant for a window of 8 chunks (the decay band of the
+// derivative operators).  Consecutive output blocks share 6 of their 8 input
+// chunks, so each wave keeps the window in registers already in MFMA operand
+// form (three BF16 limbs, or FP32), loads each chunk from memory once, splits
+// it once, and prefetches the two chunks of the next block while the MFMAs
+// of the current block run.
+template <int PREC>
+struct RingChunk;
+template <>
+struct RingChunk<kDenseBF16x3> {
+    u32x4 l0, l1, l2;
+};
+template <>
+struct RingChunk<kDenseF32> {
+    float v[8];
+};
+
+template <int PREC, typename TI>
+__device__ __forceinline__ void make_chunk(const TI (&raw)[8], RingChunk<PREC> &c)
+{
+    float x[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) x[e] = float(raw[e]);
+    if constexpr (PREC == kDenseBF16x3) {
+        split3(x, c.l0, c.l1, c.l2);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; e++) c.v[e] = x[e];
+    }
+}
+
+constexpr int kRingSlots = 8;
+
+template <int PREC, typename TI, typename TO, bool CONTIG>
+__global__ __launch_bounds__(256) void dense_ring(DenseArgs a)
+{
+    constexpr int S = kRingSlots;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    u32x4 *tl = reinterpret_cast<u32x4 *>(smem_raw);
+    constexpr int tile_u4 = PREC == kDenseBF16x3 ? 3 * 64 : 2 * 64;
+    {
+        const u32x4 *src = reinterpret_cast<const u32x4 *>(a.tiles);
+        for (int i = threadIdx.x; i < S * tile_u4; i += 256) tl[i] = src[i];
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int pl = lane & 31, g = lane >> 5;
+    const long long q0 = ((long long)blockIdx.x * 4 + wave) * 32;
+    if (q0 >= a.npencils) return;       // no barrier follows
+    const TI *__restrict__ f = reinterpret_cast<const TI *>(a.f);
+    TO *__restrict__ o = reinterpret_cast<TO *>(a.out);
+    const long long q = q0 + pl;
+    const bool valid = q < a.npencils;
+    const long long qc = valid ? q : a.npencils - 1;
+    const long long base = (qc % a.A1) * a.B1 + (qc / a.A1) * a.B2;
+    const int nc = a.n / 16, nblk = a.n / 32;
+    auto chunk_off = [&](int c) -> long long {
+        c %= nc;
+        if (c < 0) c += nc;
+        return base + (16LL * c + 8 * g) * a.stride;
+    };
+
+    RingChunk<PREC> W[S];
+    {
+        TI raw[S][8];
+#pragma unroll
+        for (int j = 0; j < S; j++) load8<TI, CONTIG>(f, chunk_off(j - a.c_off), a.stride, raw[j]);
+#pragma unroll
+        for (int j = 0; j < S; j++) make_chunk<PREC, TI>(raw[j], W[j]);
+    }
+    // Raw chunks entering the window of the next block (nx1) and of the one
+    // after (nx2): two blocks of MFMA work cover each load's latency.  The
+    // chunk index wraps, so the prefetches past the last block stay in bounds.
+    TI nx1[2][8], nx2[2][8];
+#pragma unroll
+    for (int j = 0; j < 2; j++) load8<TI, CONTIG>(f, chunk_off(2 - a.c_off + S - 2 + j), a.stride, nx1[j]);
+    for (int ib = 0; ib < nblk; ib++) {
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            load8<TI, CONTIG>(f, chunk_off(2 * (ib + 2) - a.c_off + S - 2 + j), a.stride, nx2[j]);
+        // The D tiles are re-read from LDS for every block (an opaque zero
+        // offset keeps the compiler from hoisting all 8 slots into registers,
+        // which would halve the waves per SIMD).
+        int lds_off = 0;
+        asm volatile("" : "+v"(lds_off));
+        const u32x4 *tb = tl + lds_off;
+        f32x16 hi = f32x16{}, lo = f32x16{};
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            if constexpr (PREC == kDenseBF16x3) {
+                const u32x4 a0 = tb[(s * 3 + 0) * 64 + lane];
+                const u32x4 a1 = tb[(s * 3 + 1) * 64 + lane];
+                const u32x4 a2 = tb[(s * 3 + 2) * 64 + lane];
+                hi = mfma_bf16(a0, W[s].l0, hi);
+                lo = mfma_bf16(a0, W[s].l1, lo);
+                lo = mfma_bf16(a1, W[s].l0, lo);
+                lo = mfma_bf16(a0, W[s].l2, lo);
+                lo = mfma_bf16(a1, W[s].l1, lo);
+                lo = mfma_bf16(a2, W[s].l0, lo);
+            } else {
+                const u32x4 a0 = tb[(s * 2 + 0) * 64 + lane];
+                const u32x4 a1 = tb[(s * 2 + 1) * 64 + lane];
+                const float av[8] = {__uint_as_float(a0[0]), __uint_as_float(a0[1]), __uint_as_float(a0[2]),
+                                     __uint_as_float(a0[3]), __uint_as_float(a1[0]), __uint_as_float(a1[1]),
+                                     __uint_as_float(a1[2]), __uint_as_float(a1[3])};
+#pragma unroll
+                for (int e = 0; e < 8; e += 2) {
+                    hi = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e], W[s].v[e], hi, 0, 0, 0);
+                    lo = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e + 1], W[s].v[e + 1], lo, 0, 0, 0);
+                }
+            }
+        }
+        if (valid) {
+#pragma unroll
+            for (int qd = 0; qd < 4; qd++) {
+                const long long row = 32LL * ib + 8 * qd + 4 * g;
+                store4<TO, CONTIG>(o, base + row * a.stride, a.stride, hi[4 * qd] + lo[4 * qd],
+                                   hi[4 * qd + 1] + lo[4 * qd + 1], hi[4 * qd + 2] + lo[4 * qd + 2],
+                                   hi[4 * qd + 3] + lo[4 * qd + 3]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < S - 2; j++) W[j] = W[j + 2];
+        make_chunk<PREC, TI>(nx1[0], W[S - 2]);
+        make_chunk<PREC, TI>(nx1[1], W[S - 1]);
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) nx1[j][e] = nx2[j][e];
+    }
+}
+
+template <int PREC, typename TI, typename TO>
+hipError_t dense_launch_ring(const DenseForm &D, const DenseArgs &a, bool contig, hipStream_t s)
+{
+    const size_t lds = (size_t)D.S * (PREC == kDenseBF16x3 ? 3072 : 2048);
+    const long long nblk = (a.npencils + 127) / 128;
+    if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+    auto kern = contig ? dense_ring<PREC, TI, TO, true> : dense_ring<PREC, TI, TO, false>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int PREC, typename TI, typename TO, int NPT>
+hipError_t dense_launch_n(const DenseForm &D, const DenseArgs &a, bool contig, hipStream_t s)
+{
+    const size_t lds = (size_t)D.S * (PREC == kDenseBF16x3 ? 3072 : 2048);
+    const long long per_blk = 4LL * 32 * NPT;
+    const long long nblk = (a.npencils + per_blk - 1) / per_blk;
+    if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+    auto kern = contig ? dense_apply<PREC, TI, TO, true, NPT> : dense_apply<PREC, TI, TO, false, NPT>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+// Pencil tiles per wave: two for FP32 inputs (each D operand read from LDS
+// feeds two MFMAs), one for FP64 inputs (register budget of the prefetch).
+// PADE_DENSE_NPT=1 forces one tile for FP32 inputs too.
+template <int PREC, typename TI, typename TO>
+hipError_t dense_launch_t(const DenseForm &D, const DenseArgs &a, bool contig, hipStream_t s)
+{
+    static const bool ring_ok = [] {
+        const char *e = std::getenv("PADE_DENSE_RING");
+        return !(e && std::atoi(e) == 0);
+    }();
+    if (D.S == kRingSlots && ring_ok) return dense_launch_ring<PREC, TI, TO>(D, a, contig, s);
+    if constexpr (sizeof(TI) == 4) {
+        static const int npt = [] {
+            const char *e = std::getenv("PADE_DENSE_NPT");
+            return (e && std::atoi(e) == 1) ? 1 : 2;
+        }();
+        if (npt == 2) return dense_launch_n<PREC, TI, TO, 2>(D, a, contig, s);
+    }
+    return dense_launch_n<PREC, TI, TO, 1>(D, a, contig, s);
+}
+
+template <int PREC>
+hipError_t dense_launch_p(const DenseForm &D, const DenseArgs &a, bool contig, int f_dtype, int df_dtype,
+                          hipStream_t s)
+{
+    if (f_dtype == 0)
+        return df_dtype == 0 ? dense_launch_t<PREC, double, double>(D, a, contig, s)
+                             : dense_launch_t<PREC, double, float>(D, a, contig, s);
+    return df_dtype == 0 ? dense_launch_t<PREC, float, double>(D, a, contig, s)
+                         : dense_launch_t<PREC, float, float>(D, a, contig, s);
+}
+
+hipError_t dense_launch(const DenseForm &D, int axis, const void *f, int f_dtype, void *df, int df_dtype,
+                        int na, int nb, int prec, hipStream_t s)
+{
+    DenseArgs a;
+    a.f = f;
+    a.out = df;
+    a.n = D.n;
+    a.S = D.S;
+    a.c_off = D.c_off;
+    a.npencils = (long long)na * nb;
+    const long long n = D.n;
+    if (axis == 1) {
+        a.A1 = a.npencils; a.B1 = n; a.B2 = 0; a.stride = 1;
+    } else if (axis == 2) {
+        a.A1 = na; a.B1 = 1; a.B2 = (long long)na * n; a.stride = na;
+    } else {
+        a.A1 = a.npencils; a.B1 = 1; a.B2 = 0; a.stride = (long long)na * nb;
+    }
+    const bool contig = axis == 1;
+    if (prec == kDenseBF16x3) {
+        a.tiles = D.tiles_bf;
+        return dense_launch_p<kDenseBF16x3>(D, a, contig, f_dtype, df_dtype, s);
+    }
+    a.tiles = D.tiles_f32;
+    return dense_launch_p<kDenseF32>(D, a, contig, f_dtype, df_dtype, s);
 }
 
 // ------------------------------------------------------------------------

@@ -1,0 +1,281 @@
+! Drop-in for the reference's derivatives facade (module DerivativesMod,
+! src/derivatives/derivatives.F90:14-569): per-axis method selection at init,
+! then ddx/ddy/ddz and d2dx2/d2dy2/d2dz2 on whole arrays.  On one GPU the
+! local pencil sizes of the three orientations are the global sizes, so the
+! serial init (derivatives.F90:140-186) is the one provided.  The spectral
+! methods ("four", "cheb") are outside this library and stop with 13 / 14.
+module DerivativesMod
+    use iso_c_binding
+    use padeops_c
+    use cd10stuff, only: cd10
+    use cd06stuff, only: cd06
+    implicit none
+    private
+    public :: derivatives
+
+    integer, parameter :: rkind = pade_rkind
+
+    type derivatives
+        private
+        integer :: xmethod = 0, ymethod = 0, zmethod = 0     ! 1 cd10, 2 cd06
+        type(cd10), allocatable :: xcd10, ycd10, zcd10
+        type(cd06), allocatable :: xcd06, ycd06, zcd06
+        logical :: xmetric = .false., ymetric = .false., zmetric = .false., curvilinear = .false.
+        integer :: nxg = 0, nyg = 0, nzg = 0
+        integer, dimension(3) :: xsz = 0, ysz = 0, zsz = 0
+        logical :: initialized = .false.
+    contains
+        procedure :: init
+        procedure :: destroy
+        procedure :: ddx
+        procedure :: ddy
+        procedure :: ddz
+        procedure :: d2dx2
+        procedure :: d2dy2
+        procedure :: d2dz2
+        procedure :: getMethodx
+        procedure :: getMethody
+        procedure :: getMethodz
+    end type
+
+contains
+
+    function method_name(m) result(s)
+        integer, intent(in) :: m
+        character(len=pade_clen) :: s
+        select case (m)
+        case (1)
+            s = "cd10"
+        case (2)
+            s = "cd06"
+        case default
+            s = ""
+        end select
+    end function
+
+    function getMethodx(this) result(m)
+        class(derivatives), intent(in) :: this
+        character(len=pade_clen) :: m
+        m = method_name(this%xmethod)
+    end function
+
+    function getMethody(this) result(m)
+        class(derivatives), intent(in) :: this
+        character(len=pade_clen) :: m
+        m = method_name(this%ymethod)
+    end function
+
+    function getMethodz(this) result(m)
+        class(derivatives), intent(in) :: this
+        character(len=pade_clen) :: m
+        m = method_name(this%zmethod)
+    end function
+
+    subroutine init(this, nx, ny, nz, dx, dy, dz, periodic_x, periodic_y, periodic_z, &
+                    method_x, method_y, method_z, xmetric, ymetric, zmetric, curvilinear)
+        class(derivatives), intent(inout) :: this
+        integer, intent(in) :: nx, ny, nz
+        real(rkind), intent(in) :: dx, dy, dz
+        logical, intent(in) :: periodic_x, periodic_y, periodic_z
+        character(len=*), intent(in) :: method_x, method_y, method_z
+        logical, intent(in), optional :: xmetric, ymetric, zmetric, curvilinear
+
+        if (this%initialized) then
+            write(*, '(a)') "WARNING: Reinitializing the DERIVATIVE class!"
+            call this%destroy()
+        end if
+        if (present(xmetric)) this%xmetric = xmetric
+        if (present(ymetric)) this%ymetric = ymetric
+        if (present(zmetric)) this%zmetric = zmetric
+        if (present(curvilinear)) this%curvilinear = curvilinear
+        ! derivatives.F90:239-274
+        if (this%xmetric) call pade_fail("Code INCOMPLETE! Metric terms are not supported in x direction!", 4)
+        if (this%ymetric) call pade_fail("Code INCOMPLETE! Metric terms are not supported in y direction!", 5)
+        if (this%zmetric) call pade_fail("Code INCOMPLETE! Metric terms are not supported in z direction!", 6)
+        if (this%curvilinear) call pade_fail("Code INCOMPLETE! Curvilinear Formulation is not supported!", 6)
+
+        this%nxg = nx
+        this%nyg = ny
+        this%nzg = nz
+        this%xsz = [nx, ny, nz]
+        this%ysz = this%xsz
+        this%zsz = this%xsz
+        ! derivatives.F90:276-389
+        call setup(method_x, nx, dx, periodic_x, "X", this%xmethod, this%xcd10, this%xcd06)
+        call setup(method_y, ny, dy, periodic_y, "Y", this%ymethod, this%ycd10, this%ycd06)
+        call setup(method_z, nz, dz, periodic_z, "Z", this%zmethod, this%zcd10, this%zcd06)
+        this%initialized = .true.
+    end subroutine
+
+    subroutine setup(method, n, d, periodic, axis, m, o10, o06)
+        character(len=*), intent(in) :: method, axis
+        integer, intent(in) :: n
+        real(rkind), intent(in) :: d
+        logical, intent(in) :: periodic
+        integer, intent(out) :: m
+        type(cd10), allocatable, intent(inout) :: o10
+        type(cd06), allocatable, intent(inout) :: o06
+        select case (method)
+        case ("cd10")
+            allocate(o10)
+            if (o10%init(n, d, periodic, 0, 0) /= 0) call pade_fail("Initializing cd10 failed in " // axis, 11)
+            m = 1
+        case ("cd06")
+            allocate(o06)
+            if (o06%init(n, d, periodic, 0, 0) /= 0) call pade_fail("Initializing cd06 failed in " // axis, 12)
+            m = 2
+        case ("four")
+            call pade_fail("Fourier collocation (FOUR) in " // axis // " is not provided by libpadeops_hip", 13)
+        case ("cheb")
+            call pade_fail("Chebyshev collocation (CHEB) in " // axis // " is not provided by libpadeops_hip", 14)
+        case default
+            call pade_fail("Invalid method selected in " // axis // " direction", 1)
+        end select
+    end subroutine
+
+    ! derivatives.F90:391-445
+    subroutine destroy(this)
+        class(derivatives), intent(inout) :: this
+        if (allocated(this%xcd10)) then
+            call this%xcd10%destroy()
+            deallocate(this%xcd10)
+        end if
+        if (allocated(this%ycd10)) then
+            call this%ycd10%destroy()
+            deallocate(this%ycd10)
+        end if
+        if (allocated(this%zcd10)) then
+            call this%zcd10%destroy()
+            deallocate(this%zcd10)
+        end if
+        if (allocated(this%xcd06)) then
+            call this%xcd06%destroy()
+            deallocate(this%xcd06)
+        end if
+        if (allocated(this%ycd06)) then
+            call this%ycd06%destroy()
+            deallocate(this%ycd06)
+        end if
+        if (allocated(this%zcd06)) then
+            call this%zcd06%destroy()
+            deallocate(this%zcd06)
+        end if
+        this%xmethod = 0
+        this%ymethod = 0
+        this%zmethod = 0
+        this%xmetric = .false.
+        this%ymetric = .false.
+        this%zmetric = .false.
+        this%curvilinear = .false.
+        this%initialized = .false.
+    end subroutine
+
+    ! derivatives.F90:447-468
+    subroutine ddx(this, f, dfdx, bc1, bcn)
+        class(derivatives), intent(in) :: this
+        real(rkind), intent(in), dimension(this%xsz(1), this%xsz(2), this%xsz(3)) :: f
+        real(rkind), intent(out), dimension(this%xsz(1), this%xsz(2), this%xsz(3)) :: dfdx
+        integer, optional, intent(in) :: bc1, bcn
+        select case (this%xmethod)
+        case (1)
+            if (present(bc1) .and. present(bcn)) then
+                call this%xcd10%dd1(f, dfdx, this%xsz(2), this%xsz(3), bc1, bcn)
+            else
+                call this%xcd10%dd1(f, dfdx, this%xsz(2), this%xsz(3))
+            end if
+        case (2)
+            call this%xcd06%dd1(f, dfdx, this%xsz(2), this%xsz(3))
+        case default
+            call pade_fail("derivatives used before init", 400)
+        end select
+    end subroutine
+
+    ! derivatives.F90:470-491
+    subroutine ddy(this, f, dfdx, bc1, bcn)
+        class(derivatives), intent(in) :: this
+        real(rkind), intent(in), dimension(this%ysz(1), this%ysz(2), this%ysz(3)) :: f
+        real(rkind), intent(out), dimension(this%ysz(1), this%ysz(2), this%ysz(3)) :: dfdx
+        integer, optional, intent(in) :: bc1, bcn
+        select case (this%ymethod)
+        case (1)
+            if (present(bc1) .and. present(bcn)) then
+                call this%ycd10%dd2(f, dfdx, this%ysz(1), this%ysz(3), bc1, bcn)
+            else
+                call this%ycd10%dd2(f, dfdx, this%ysz(1), this%ysz(3))
+            end if
+        case (2)
+            call this%ycd06%dd2(f, dfdx, this%ysz(1), this%ysz(3))
+        case default
+            call pade_fail("derivatives used before init", 400)
+        end select
+    end subroutine
+
+    ! derivatives.F90:493-513
+    subroutine ddz(this, f, dfdx, bc1, bcn)
+        class(derivatives), intent(in) :: this
+        real(rkind), intent(in), dimension(this%zsz(1), this%zsz(2), this%zsz(3)) :: f
+        real(rkind), intent(out), dimension(this%zsz(1), this%zsz(2), this%zsz(3)) :: dfdx
+        integer, optional, intent(in) :: bc1, bcn
+        select case (this%zmethod)
+        case (1)
+            if (present(bc1) .and. present(bcn)) then
+                call this%zcd10%dd3(f, dfdx, this%zsz(1), this%zsz(2), bc1, bcn)
+            else
+                call this%zcd10%dd3(f, dfdx, this%zsz(1), this%zsz(2))
+            end if
+        case (2)
+            call this%zcd06%dd3(f, dfdx, this%zsz(1), this%zsz(2))
+        case default
+            call pade_fail("derivatives used before init", 400)
+        end select
+    end subroutine
+
+    ! derivatives.F90:515-532
+    subroutine d2dx2(this, f, d2fdx2, bc1, bcn)
+        class(derivatives), intent(in) :: this
+        real(rkind), intent(in), dimension(this%xsz(1), this%xsz(2), this%xsz(3)) :: f
+        real(rkind), intent(out), dimension(this%xsz(1), this%xsz(2), this%xsz(3)) :: d2fdx2
+        integer, optional, intent(in) :: bc1, bcn
+        select case (this%xmethod)
+        case (1)
+            call this%xcd10%d2d1(f, d2fdx2, this%xsz(2), this%xsz(3), bc1, bcn)
+        case (2)
+            call pade_fail("CD06 is incomplete right now", 21)
+        case default
+            call pade_fail("derivatives used before init", 400)
+        end select
+    end subroutine
+
+    ! derivatives.F90:534-551
+    subroutine d2dy2(this, f, d2fdx2, bc1, bcn)
+        class(derivatives), intent(in) :: this
+        real(rkind), intent(in), dimension(this%ysz(1), this%ysz(2), this%ysz(3)) :: f
+        real(rkind), intent(out), dimension(this%ysz(1), this%ysz(2), this%ysz(3)) :: d2fdx2
+        integer, optional, intent(in) :: bc1, bcn
+        select case (this%ymethod)
+        case (1)
+            call this%ycd10%d2d2(f, d2fdx2, this%ysz(1), this%ysz(3), bc1, bcn)
+        case (2)
+            call pade_fail("CD06 is incomplete right now", 21)
+        case default
+            call pade_fail("derivatives used before init", 400)
+        end select
+    end subroutine
+
+    ! derivatives.F90:553-569
+    subroutine d2dz2(this, f, d2fdx2, bc1, bcn)
+        class(derivatives), intent(in) :: this
+        real(rkind), intent(in), dimension(this%zsz(1), this%zsz(2), this%zsz(3)) :: f
+        real(rkind), intent(out), dimension(this%zsz(1), this%zsz(2), this%zsz(3)) :: d2fdx2
+        integer, optional, intent(in) :: bc1, bcn
+        select case (this%zmethod)
+        case (1)
+            call this%zcd10%d2d3(f, d2fdx2, this%zsz(1), this%zsz(2), bc1, bcn)
+        case (2)
+            call pade_fail("CD06 is incomplete right now", 21)
+        case default
+            call pade_fail("derivatives used before init", 400)
+        end select
+    end subroutine
+
+end module DerivativesMod

@@ -1,0 +1,149 @@
+! Drop-in for the reference's filters facade (module FiltersMod,
+! src/filters/filters.F90:13-322): per-axis filter selection at init, then
+! filterx/filtery/filterz on whole arrays (serial init, filters.F90:299-322).
+! The gaussian and least-squares filters are outside this library and stop
+! with 52.
+module FiltersMod
+    use iso_c_binding
+    use padeops_c
+    use cf90stuff, only: cf90
+    implicit none
+    private
+    public :: filters
+
+    integer, parameter :: rkind = pade_rkind
+
+    type filters
+        private
+        integer :: xmethod = 0, ymethod = 0, zmethod = 0    ! 1 cf90
+        type(cf90), allocatable :: xcf90, ycf90, zcf90
+        integer, dimension(3) :: xsz = 0, ysz = 0, zsz = 0
+        logical :: initialized = .false.
+    contains
+        procedure :: init
+        procedure :: destroy
+        procedure :: filterx
+        procedure :: filtery
+        procedure :: filterz
+        procedure :: getMethodx
+        procedure :: getMethody
+        procedure :: getMethodz
+    end type
+
+contains
+
+    function method_name(m) result(s)
+        integer, intent(in) :: m
+        character(len=pade_clen) :: s
+        s = ""
+        if (m == 1) s = "cf90"
+    end function
+
+    function getMethodx(this) result(m)
+        class(filters), intent(in) :: this
+        character(len=pade_clen) :: m
+        m = method_name(this%xmethod)
+    end function
+
+    function getMethody(this) result(m)
+        class(filters), intent(in) :: this
+        character(len=pade_clen) :: m
+        m = method_name(this%ymethod)
+    end function
+
+    function getMethodz(this) result(m)
+        class(filters), intent(in) :: this
+        character(len=pade_clen) :: m
+        m = method_name(this%zmethod)
+    end function
+
+    subroutine init(this, nx, ny, nz, periodicx, periodicy, periodicz, methodx, methody, methodz)
+        class(filters), intent(inout) :: this
+        integer, intent(in) :: nx, ny, nz
+        logical, intent(in) :: periodicx, periodicy, periodicz
+        character(len=*), intent(in) :: methodx, methody, methodz
+        if (this%initialized) then
+            write(*, '(a)') "WARNING: Reinitializing the FILTER class!"
+            call this%destroy()
+        end if
+        this%xsz = [nx, ny, nz]
+        this%ysz = this%xsz
+        this%zsz = this%xsz
+        ! filters.F90:82-184
+        call setup(methodx, nx, periodicx, "X", this%xmethod, this%xcf90)
+        call setup(methody, ny, periodicy, "Y", this%ymethod, this%ycf90)
+        call setup(methodz, nz, periodicz, "Z", this%zmethod, this%zcf90)
+        this%initialized = .true.
+    end subroutine
+
+    subroutine setup(method, n, periodic, axis, m, o)
+        character(len=*), intent(in) :: method, axis
+        integer, intent(in) :: n
+        logical, intent(in) :: periodic
+        integer, intent(out) :: m
+        type(cf90), allocatable, intent(inout) :: o
+        select case (method)
+        case ("cf90")
+            allocate(o)
+            if (o%init(n, periodic) /= 0) call pade_fail("Initializing cf90 failed in " // axis // " ", 51)
+            m = 1
+        case ("gaussian", "lstsq")
+            call pade_fail("the " // method // " filter in " // axis // " is not provided by libpadeops_hip", 52)
+        case default
+            call pade_fail("Incorrect method select in direction " // axis, 52)
+        end select
+    end subroutine
+
+    ! filters.F90:186-217; components are released here, so init after
+    ! destroy works
+    subroutine destroy(this)
+        class(filters), intent(inout) :: this
+        if (allocated(this%xcf90)) then
+            call this%xcf90%destroy()
+            deallocate(this%xcf90)
+        end if
+        if (allocated(this%ycf90)) then
+            call this%ycf90%destroy()
+            deallocate(this%ycf90)
+        end if
+        if (allocated(this%zcf90)) then
+            call this%zcf90%destroy()
+            deallocate(this%zcf90)
+        end if
+        this%xmethod = 0
+        this%ymethod = 0
+        this%zmethod = 0
+        this%initialized = .false.
+    end subroutine
+
+    ! filters.F90:220-235
+    subroutine filterx(this, f, ff, bc1, bcn)
+        class(filters), intent(in) :: this
+        real(rkind), dimension(this%xsz(1), this%xsz(2), this%xsz(3)), intent(in) :: f
+        real(rkind), dimension(this%xsz(1), this%xsz(2), this%xsz(3)), intent(out) :: ff
+        integer, optional, intent(in) :: bc1, bcn
+        if (this%xmethod /= 1) call pade_fail("filters used before init", 400)
+        call this%xcf90%filter1(f, ff, this%xsz(2), this%xsz(3), bc1, bcn)
+    end subroutine
+
+    ! filters.F90:237-252
+    subroutine filtery(this, f, ff, bc1, bcn)
+        class(filters), intent(in) :: this
+        real(rkind), dimension(this%ysz(1), this%ysz(2), this%ysz(3)), intent(in) :: f
+        real(rkind), dimension(this%ysz(1), this%ysz(2), this%ysz(3)), intent(out) :: ff
+        integer, optional, intent(in) :: bc1, bcn
+        if (this%ymethod /= 1) call pade_fail("filters used before init", 400)
+        call this%ycf90%filter2(f, ff, this%ysz(1), this%ysz(3), bc1, bcn)
+    end subroutine
+
+    ! filters.F90:254-269
+    subroutine filterz(this, f, ff, bc1, bcn)
+        class(filters), intent(in) :: this
+        real(rkind), dimension(this%zsz(1), this%zsz(2), this%zsz(3)), intent(in) :: f
+        real(rkind), dimension(this%zsz(1), this%zsz(2), this%zsz(3)), intent(out) :: ff
+        integer, optional, intent(in) :: bc1, bcn
+        if (this%zmethod /= 1) call pade_fail("filters used before init", 400)
+        call this%zcf90%filter3(f, ff, this%zsz(1), this%zsz(2), bc1, bcn)
+    end subroutine
+
+end module FiltersMod

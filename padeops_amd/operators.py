@@ -143,16 +143,34 @@ class _CompactOp:
 
     def dense_apply(self, oper, axis, f, df, na, nb, prec, bc1=None, bcn=None):
         """Dense-operator MFMA form (precision study); f/df float64 or float32."""
+        if not self._h:
+            raise PadeError(400, "operator used before init")
         lib = _lib.load()
         count = self._n * int(na) * int(nb)
-        if f.numel() != count or df.numel() != count:
-            raise ValueError("array size does not match n*na*nb")
         dt = {torch.float64: _lib.PADE_F64, torch.float32: _lib.PADE_F32}
+        if f.dtype not in dt or df.dtype not in dt:
+            raise TypeError("f and df must be float64 or float32")
+        if not (f.is_cuda and df.is_cuda):
+            raise TypeError("device arrays must live on the GPU")
+        if f.numel() != count or df.numel() != count:
+            raise ValueError(f"array size does not match n*na*nb = {count}")
+        if not (_is_dense_block(f) and _is_dense_block(df)):
+            raise ValueError("arrays must be dense blocks in Fortran order")
+        if f.data_ptr() == df.data_ptr():
+            raise ValueError("f and df must not alias")
         _lib.check(lib.pade_op_dense_apply(self._h, oper, axis, ctypes.c_void_p(f.data_ptr()), dt[f.dtype],
                                            ctypes.c_void_p(df.data_ptr()), dt[df.dtype], int(na), int(nb),
                                            0 if bc1 is None else int(bc1), 0 if bcn is None else int(bcn),
                                            int(prec), _stream_ptr()))
         return df
+
+    def dense_info(self, oper, prec):
+        """(window, halfwidth) of the dense form: input points visited per
+        output block and the decay half-width kept."""
+        lib = _lib.load()
+        w, h = ctypes.c_int(), ctypes.c_int()
+        _lib.check(lib.pade_op_dense_info(self._h, int(oper), int(prec), ctypes.byref(w), ctypes.byref(h)))
+        return w.value, h.value
 
 
 class cd06(_CompactOp):

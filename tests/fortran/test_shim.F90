@@ -1,0 +1,136 @@
+! Exercises the Fortran drop-in modules (cd06stuff, cd10stuff, cf90stuff,
+! DerivativesMod, FiltersMod) the way the reference's callers use them, and
+! writes inputs and outputs to a stream file that tests/test_gpu_fortran.py
+! checks against the oracle.  With the argument "badbc" it calls cd10%dd1
+! with bc1_ = 5, which must stop with code 324.
+program test_shim
+    use cd10stuff, only: cd10
+    use cd06stuff, only: cd06
+    use cf90stuff, only: cf90
+    use DerivativesMod, only: derivatives
+    use FiltersMod, only: filters
+    implicit none
+    integer, parameter :: rk = kind(0.d0)
+    integer, parameter :: n1 = 24, n2 = 20, n3 = 16
+    real(rk), parameter :: pi = 4.0_rk * atan(1.0_rk)
+    real(rk), allocatable :: f(:, :, :), g(:, :, :), p1(:)
+    character(len=512) :: path
+    integer :: u, i, j, k, ierr
+    type(cd10) :: c10
+    type(cd06) :: c06
+    type(cf90) :: c90
+    type(derivatives) :: der
+    type(filters) :: fil
+
+    call get_command_argument(1, path)
+    if (trim(path) == "badbc") then
+        allocate(f(n1, n2, n3), g(n1, n2, n3))
+        f = 1.0_rk
+        ierr = c10%init(n1, 0.1_rk, .false., 0, 0)
+        call c10%dd1(f, g, n2, n3, 5, 0)
+        print '(a)', "not reached"
+        stop 1
+    end if
+
+    allocate(f(n1, n2, n3), g(n1, n2, n3), p1(n1))
+    do k = 1, n3
+        do j = 1, n2
+            do i = 1, n1
+                f(i, j, k) = sin(0.37_rk * i + 0.71_rk * j * j) * cos(0.13_rk * k * i) + 1.0e-2_rk * i * k
+            end do
+        end do
+    end do
+    open(newunit=u, file=trim(path), access="stream", form="unformatted", status="replace")
+    call put("input", f)
+
+    ierr = c10%init(n1, 2 * pi / n1, .true., 0, 0)
+    call c10%dd1(f, g, n2, n3)
+    call put("cd10_p_dd1", g)
+    ierr = c10%init(n2, 2 * pi / n2, .true., 0, 0)
+    call c10%dd2(f, g, n1, n3)
+    call put("cd10_p_dd2", g)
+    ierr = c10%init(n3, 2 * pi / n3, .true., 0, 0)
+    call c10%dd3(f, g, n1, n2)
+    call put("cd10_p_dd3", g)
+    call c10%d2d3(f, g, n1, n2)
+    call put("cd10_p_d2d3", g)
+    ierr = c10%init(n3, 1.0_rk / (n3 - 1), .false., 0, 0)
+    call c10%d2d3(f, g, n1, n2, 1, -1)
+    call put("cd10_b_d2d3_1m1", g)
+    ierr = c10%init(n1, 1.0_rk / (n1 - 1), .false., 0, 0)
+    call c10%dd1(f, g, n2, n3)
+    call put("cd10_b_dd1_00", g)
+    call c10%destroy()
+
+    ierr = c06%init(n2, 2 * pi / n2, .true., 0, 0)
+    call c06%dd2(f, g, n1, n3)
+    call put("cd06_p_dd2", g)
+    ierr = c06%init(n3, 1.0_rk / (n3 - 1), .false., 0, 0)
+    call c06%dd3(f, g, n1, n2)
+    call put("cd06_b_dd3", g)
+    ierr = c06%init(n1, 2 * pi / n1, .true., 0, 0)
+    p1 = c06%cd06der2(f(:, 3, 5))
+    g = 0.0_rk
+    g(:, 1, 1) = p1
+    call put("cd06_p_der2_pencil", g)
+    call c06%destroy()
+
+    ierr = c90%init(n1, .false.)
+    call c90%filter1(f, g, n2, n3, 0, 1)
+    call put("cf90_b_f1_01", g)
+    ierr = c90%init(n3, .true.)
+    call c90%filter3(f, g, n1, n2)
+    call put("cf90_p_f3", g)
+    call c90%destroy()
+
+    call der%init(n1, n2, n3, 2 * pi / n1, 1.0_rk / (n2 - 1), 1.0_rk / (n3 - 1), .true., .false., .false., &
+                  "cd10", "cd06", "cd10")
+    call der%ddx(f, g)
+    call put("fac_ddx", g)
+    call der%ddy(f, g)
+    call put("fac_ddy", g)
+    call der%ddz(f, g, -1, 1)
+    call put("fac_ddz_m11", g)
+    call der%d2dz2(f, g)
+    call put("fac_d2dz2", g)
+    call der%destroy()
+
+    call fil%init(n1, n2, n3, .true., .false., .false., "cf90", "cf90", "cf90")
+    call fil%filtery(f, g, 1, 1)
+    call put("fil_filtery_11", g)
+    call fil%destroy()
+    call fil%init(n1, n2, n3, .true., .true., .false., "cf90", "cf90", "cf90")
+    call fil%filterz(f, g)
+    call put("fil_filterz", g)
+
+    ! init error codes (the reference's ierr values)
+    ierr = c10%init(4, 0.1_rk, .true., 0, 0)
+    call put_int("ierr_cd10_n4", ierr)
+    ierr = c06%init(4, 0.1_rk, .true., 0, 0)
+    call put_int("ierr_cd06_n4p", ierr)
+    ierr = c06%init(6, 0.1_rk, .false., 0, 0)
+    call put_int("ierr_cd06_n6b", ierr)
+    ierr = c90%init(5, .true.)
+    call put_int("ierr_cf90_n5", ierr)
+    close(u)
+    print '(a)', "fortran shim: done"
+
+contains
+
+    subroutine put(name, a)
+        character(len=*), intent(in) :: name
+        real(rk), intent(in) :: a(:, :, :)
+        character(len=16) :: tag
+        tag = name
+        write(u) tag, int(size(a, 1), 4), int(size(a, 2), 4), int(size(a, 3), 4), a
+    end subroutine
+
+    subroutine put_int(name, v)
+        character(len=*), intent(in) :: name
+        integer, intent(in) :: v
+        character(len=16) :: tag
+        tag = name
+        write(u) tag, int(-1, 4), int(v, 4), int(0, 4)
+    end subroutine
+
+end program test_shim

@@ -1,0 +1,104 @@
+"""The Fortran drop-in modules (padeops_amd/fortran/*.F90) driven by a Fortran
+program (tests/fortran/test_shim.F90) exactly as the reference's callers use
+cd06 / cd10 / cf90 / derivatives / filters; its outputs are checked against
+the oracle to 1e-12, its init codes against the reference's (2, 3, 7), and a
+bad boundary code must stop the program with 324 (cd10.F90:2042-2058)."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import pade_dense as pd
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(os.path.dirname(__file__), "fortran", "bin", "test_shim")
+
+
+def read_records(path):
+    out = {}
+    with open(path, "rb") as fh:
+        data = fh.read()
+    pos = 0
+    while pos < len(data):
+        tag = data[pos:pos + 16].decode().strip()
+        n1, n2, n3 = np.frombuffer(data[pos + 16:pos + 28], dtype=np.int32)
+        pos += 28
+        if n1 == -1:
+            out[tag] = int(n2)
+            continue
+        cnt = int(n1) * int(n2) * int(n3)
+        arr = np.frombuffer(data[pos:pos + 8 * cnt], dtype=np.float64).reshape((n1, n2, n3), order="F")
+        out[tag] = arr
+        pos += 8 * cnt
+    return out
+
+
+def ref(kind, which, f, axis, periodic, bc=(0, 0)):
+    n = f.shape[axis]
+    dx = 2 * math.pi / n if periodic else 1.0 / (n - 1)
+    D = pd.dense_operator(kind, which, n, dx, periodic, bc[0], bc[1])
+    return pd.apply_axis(D, f, axis)
+
+
+@pytest.fixture(scope="module")
+def records(tmp_path_factory):
+    if not os.path.exists(EXE):
+        pytest.fail("tests/fortran/bin/test_shim is not built (run padeops_amd._build)")
+    path = str(tmp_path_factory.mktemp("fshim") / "out.bin")
+    r = subprocess.run([EXE, path], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return read_records(path)
+
+
+CASES = [
+    ("cd10_p_dd1", "cd10", "d1", 0, True, (0, 0)),
+    ("cd10_p_dd2", "cd10", "d1", 1, True, (0, 0)),
+    ("cd10_p_dd3", "cd10", "d1", 2, True, (0, 0)),
+    ("cd10_p_d2d3", "cd10", "d2", 2, True, (0, 0)),
+    ("cd10_b_d2d3_1m1", "cd10", "d2", 2, False, (1, -1)),
+    ("cd10_b_dd1_00", "cd10", "d1", 0, False, (0, 0)),
+    ("cd06_p_dd2", "cd06", "d1", 1, True, (0, 0)),
+    ("cd06_b_dd3", "cd06", "d1", 2, False, (0, 0)),
+    ("cf90_b_f1_01", "cf90", "filter", 0, False, (0, 1)),
+    ("cf90_p_f3", "cf90", "filter", 2, True, (0, 0)),
+    ("fac_ddx", "cd10", "d1", 0, True, (0, 0)),
+    ("fac_ddy", "cd06", "d1", 1, False, (0, 0)),
+    ("fac_ddz_m11", "cd10", "d1", 2, False, (-1, 1)),
+    ("fac_d2dz2", "cd10", "d2", 2, False, (0, 0)),
+    ("fil_filtery_11", "cf90", "filter", 1, False, (1, 1)),
+    ("fil_filterz", "cf90", "filter", 2, False, (0, 0)),
+]
+
+
+@pytest.mark.parametrize("tag,kind,which,axis,periodic,bc", CASES)
+def test_fortran_shim_outputs(records, tag, kind, which, axis, periodic, bc):
+    f = records["input"]
+    expect = ref(kind, which, f, axis, periodic, bc)
+    got = records[tag]
+    err = float(np.abs(got - expect).max() / np.abs(expect).max())
+    assert err < 1e-12, (tag, err)
+
+
+def test_fortran_cd06der2_pencil(records):
+    f = records["input"][:, 2, 4]
+    n = f.shape[0]
+    D = pd.dense_operator("cd06", "d2", n, 2 * math.pi / n, True, 0, 0)
+    expect = D @ f
+    got = records["cd06_p_der2_pencil"][:, 0, 0]
+    assert float(np.abs(got - expect).max() / np.abs(expect).max()) < 1e-12
+
+
+def test_fortran_init_codes(records):
+    assert records["ierr_cd10_n4"] == 2
+    assert records["ierr_cd06_n4p"] == 3
+    assert records["ierr_cd06_n6b"] == 2
+    assert records["ierr_cf90_n5"] == 7
+
+
+def test_fortran_bad_bc_stops_324():
+    r = subprocess.run([EXE, "badbc"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 324 % 256, (r.returncode, r.stdout, r.stderr)
+    assert "Incorrect boundary specification" in r.stdout + r.stderr

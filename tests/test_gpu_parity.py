@@ -217,3 +217,60 @@ def test_golden_fixtures():
         bc = None if c["kind"] == "cd06" else c["bc"]
         out = run(op, c["kind"], c["which"], c["axis"], c["f"], bc=bc)
         assert relerr(out, c["y"]) < TOL64, (c["name"], relerr(out, c["y"]))
+
+
+# ---------------------------------------------------------------- dense form
+DENSE_TOL = 1e-6      # acceptance bar of the MFMA forms (normwise, vs FP64)
+
+
+def run_dense(op, which, axis, f, prec, in_dtype, out_dtype):
+    shape = f.shape
+    ft = po.as_fortran(torch.tensor(f, device=DEV).to(in_dtype))
+    out = po.fortran_empty(shape, dtype=out_dtype, device=DEV)
+    na, nb = [s for i, s in enumerate(shape) if i != axis - 1]
+    oper = {"d1": _lib.PADE_D1, "d2": _lib.PADE_D2, "filter": _lib.PADE_FILTER}[which]
+    op.dense_apply(oper, axis, ft, out, na, nb, prec)
+    torch.cuda.synchronize()
+    return out.double().cpu().numpy()
+
+
+@pytest.mark.parametrize("kind,which", OPS)
+@pytest.mark.parametrize("prec", [_lib.PADE_DENSE_F32, _lib.PADE_DENSE_BF16X3,
+                                  _lib.PADE_DENSE_BF16X3 | _lib.PADE_DENSE_FULL])
+@pytest.mark.parametrize("n,axis", [(64, 1), (128, 2), (256, 3), (192, 1), (512, 1)])
+def test_dense_forms(kind, which, prec, n, axis):
+    dx = 2 * np.pi / n
+    f = field(shape_for(axis, n, 40, 7), seed=3 * n + axis)
+    op = make(kind, n, dx, True)
+    ref = ref_apply(kind, which, f, axis, dx, True, 0, 0)
+    for in_dt, out_dt in ((torch.float64, torch.float64), (torch.float32, torch.float32)):
+        out = run_dense(op, which, axis, f, prec, in_dt, out_dt)
+        err = relerr(out, ref)
+        assert err < DENSE_TOL, (kind, which, prec, n, axis, in_dt, err)
+
+
+@pytest.mark.parametrize("axis", [1, 2, 3])
+def test_dense_mixed_dtypes_and_ragged(axis):
+    n = 128
+    dx = 2 * np.pi / n
+    op = make("cd10", n, dx, True)
+    f = field(shape_for(axis, n, 37, 3), seed=11)      # 111 pencils: ragged tiles
+    ref = ref_apply("cd10", "d1", f, axis, dx, True, 0, 0)
+    for prec in (_lib.PADE_DENSE_F32, _lib.PADE_DENSE_BF16X3, _lib.PADE_DENSE_F32 | _lib.PADE_DENSE_FULL):
+        for in_dt, out_dt in ((torch.float64, torch.float32), (torch.float32, torch.float64)):
+            err = relerr(run_dense(op, "d1", axis, f, prec, in_dt, out_dt), ref)
+            assert err < DENSE_TOL, (prec, in_dt, out_dt, err)
+
+
+def test_dense_not_provided():
+    f = torch.zeros(64 * 4 * 4, dtype=torch.float64, device=DEV)
+    g = torch.zeros_like(f)
+    op = make("cd10", 64, 0.1, False)
+    with pytest.raises(po.PadeError) as e:
+        op.dense_apply(_lib.PADE_D1, 1, f, g, 4, 4, _lib.PADE_DENSE_F32)
+    assert e.value.code == 21
+    op = make("cd10", 96, 0.1, True)
+    f = torch.zeros(96 * 4 * 4, dtype=torch.float64, device=DEV)
+    with pytest.raises(po.PadeError) as e:
+        op.dense_apply(_lib.PADE_D1, 1, f, torch.zeros_like(f), 4, 4, _lib.PADE_DENSE_F32)
+    assert e.value.code == 21

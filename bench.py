@@ -85,6 +85,16 @@ def cpu_baseline(n, dx, nthreads):
     return res, f"{n}x{na}x{nb} slab of pencils (x-derivative), median of repeated applies"
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def committed_traffic(n):
     """HBM bytes per launch of the dominant kernel from the committed counter
     summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings)."""
@@ -161,6 +171,30 @@ def main():
         ms_z = timed(lambda: op.dd3(f, df, n, n), args.steps, args.warmup)
         extra["fp64_gpts"] = {"x": value, "y": pts / (ms_y * 1e-3) / 1e9, "z": pts / (ms_z * 1e-3) / 1e9,
                               "xyz_aggregate": pts / ((ms + ms_y + ms_z) * 1e-3) / 1e9}
+        # every operator of the path at the same size (FP64, one launch per step)
+        log("bench: operator table")
+        table = {}
+        specs = [("cd06", "d1", True, None), ("cd06", "d2", True, None), ("cd10", "d2", True, None),
+                 ("cf90", "filter", True, None), ("cd06", "d1", False, None), ("cd10", "d1", False, (0, 0)),
+                 ("cd10", "d1", False, (1, -1)), ("cd10", "d2", False, (0, 0)), ("cf90", "filter", False, (0, 0))]
+        opers = {"d1": _lib.PADE_D1, "d2": _lib.PADE_D2, "filter": _lib.PADE_FILTER}
+        for kind, which, periodic, bc in specs:
+            o = {"cd06": po.cd06, "cd10": po.cd10, "cf90": po.cf90}[kind]()
+            dxx = dx if periodic else 1.0 / (n - 1)
+            assert (o.init(n, periodic) if kind == "cf90" else o.init(n, dxx, periodic)) == 0
+            b1, bn = (None, None) if bc is None else bc
+            row = {}
+            for axis in (1, 3):
+                for adj in (False, True):
+                    t = timed(lambda: o._apply(opers[which], axis, f, df, n, n, b1, bn, adjoint=adj),
+                              max(3, args.steps // 5), 2)
+                    row[("xyz"[axis - 1]) + ("_adj" if adj else "")] = round(pts / (t * 1e-3) / 1e9, 1)
+            table[f"{kind}_{which}_{'periodic' if periodic else 'bounded'}" + ("" if bc is None else f"_bc{bc[0]}{bc[1]}")] = row
+            o.destroy()
+        extra["fp64_gpts_by_operator"] = table
+        # practical ceiling on this machine: a plain device copy of the same field
+        ms_copy = timed(lambda: df.copy_(f), args.steps, args.warmup)
+        extra["device_copy_gbs"] = 16.0 * pts / (ms_copy * 1e-3) / 1e9
         log("bench: FP32 banded")
         f32 = po.fortran_empty((n, n, n), dtype=torch.float32, device=dev)
         f32.copy_(f)
@@ -210,7 +244,7 @@ def main():
         nthreads = max(1, min(16, os.cpu_count() or 1))
         res, sample = cpu_baseline(n, dx, nthreads)
         cpu = {"value": res[nthreads], "unit": "Gpts/s", "cores": nthreads, "kind": "port",
-               "sample": sample, "single_thread_gpts": res[1]}
+               "sample": sample, "single_thread_gpts": res[1], "cpu_model": cpu_model()}
 
     line = {
         "metric": "Gpts/s per CD10 3D derivative at 512^3: FP64 banded vs MFMA form at <=1e-6 err",

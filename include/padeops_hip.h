@@ -77,6 +77,16 @@ int pade_op_apply_adjoint(pade_op_t op, int oper, int axis, const double *f, dou
 int pade_op_apply_adjoint_f32(pade_op_t op, int oper, int axis, const float *f, float *df, int na,
                               int nb, int bc1, int bcn, void *stream);
 
+/* Apply with an accumulate epilogue: df = alpha * op(f) + beta * df, in the
+ * same pass over df (the composite operators gradient / divergence / curl of
+ * src/utilities/operators.F90:18-156 sum derivatives along several axes;
+ * this fuses each sum into the producing kernel instead of a separate array
+ * add).  No reference counterpart. */
+int pade_op_apply_axpby(pade_op_t op, int oper, int axis, double alpha, const double *f, double beta, double *df,
+                        int na, int nb, int bc1, int bcn, void *stream);
+int pade_op_apply_axpby_f32(pade_op_t op, int oper, int axis, float alpha, const float *f, float beta, float *df,
+                            int na, int nb, int bc1, int bcn, void *stream);
+
 /* Host-array apply for drop-in callers whose data lives in host memory:
  * copies in, applies on the device, copies out, synchronous.  Same meaning as
  * pade_op_apply (the reference's dd*, d2d*, filter* take host arrays). */

@@ -19,7 +19,8 @@ PADE_F64, PADE_F32 = 0, 1
 # every symbol include/padeops_hip.h declares
 EXPORTS = [
     "pade_op_create", "pade_op_destroy", "pade_op_size", "pade_op_apply", "pade_op_apply_f32",
-    "pade_op_apply_adjoint", "pade_op_apply_adjoint_f32", "pade_op_apply_host",
+    "pade_op_apply_adjoint", "pade_op_apply_adjoint_f32", "pade_op_apply_axpby", "pade_op_apply_axpby_f32",
+    "pade_op_apply_host",
     "pade_op_dense_apply", "pade_op_dense_info", "pade_pencil_copy", "pade_permute3", "pade_plan_export",
     "pade_last_error", "pade_version",
 ]
@@ -52,6 +53,10 @@ def load():
     apply_sig = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p]
     for name in ("pade_op_apply", "pade_op_apply_f32", "pade_op_apply_adjoint", "pade_op_apply_adjoint_f32"):
         getattr(lib, name).argtypes = apply_sig
+    lib.pade_op_apply_axpby.argtypes = [c_void_p, c_int, c_int, c_double, c_void_p, c_double, c_void_p, c_int,
+                                        c_int, c_int, c_int, c_void_p]
+    lib.pade_op_apply_axpby_f32.argtypes = [c_void_p, c_int, c_int, ctypes.c_float, c_void_p, ctypes.c_float,
+                                            c_void_p, c_int, c_int, c_int, c_int, c_void_p]
     lib.pade_op_apply_host.argtypes = [c_void_p, c_int, c_int, dp, dp, c_int, c_int, c_int, c_int]
     lib.pade_op_dense_apply.argtypes = [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_int,
                                         c_int, c_int, c_int, c_int, c_void_p]

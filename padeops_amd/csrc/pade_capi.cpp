@@ -263,13 +263,20 @@ const Variant *pick_variant(const Plan &pl, long long npencils, int mode, int *p
     return best;
 }
 
+// Optional epilogue of an apply: df = alpha * op(f) + beta * df.
+template <typename T>
+struct Epilogue {
+    bool on = false;
+    T alpha = T(1), beta = T(0);
+};
+
 template <typename T>
 int apply_impl(pade_op_t op, int oper, int axis, const T *f, T *df, int na, int nb, int bc1, int bcn,
-               bool adj, hipStream_t s);
+               bool adj, hipStream_t s, Epilogue<T> epi = Epilogue<T>());
 
 template <typename T>
 int apply_one(pade_op_t op, const Plan &pl, int axis, const T *f, T *df, int na, int nb, bool adj,
-              hipStream_t s)
+              hipStream_t s, Epilogue<T> epi)
 {
     const int n = op->n;
     pade::ApplyArgs<T> a{};
@@ -303,6 +310,9 @@ int apply_one(pade_op_t op, const Plan &pl, int axis, const T *f, T *df, int na,
     a.ncls = v->ncls;
     a.kq = v->kq;
     for (int k = 0; k < pade::kMaxPartitions; k++) a.cls[k] = v->cls[k];
+    a.axpby = 0;
+    a.alpha = T(1);
+    a.beta = T(0);
     // Periodic operators are circulant, so A^T and B^T commute and the adjoint
     // B^T A^{-T} equals A^{-T} B^T: the transposed tables run through the
     // forward kernel (stencil first).  This avoids amplifying the Nyquist band
@@ -311,24 +321,42 @@ int apply_one(pade_op_t op, const Plan &pl, int axis, const T *f, T *df, int na,
     const bool kadj = adj && !op->periodic;
     hipError_t e;
     if (!use_generic) {
+        if (epi.on) {
+            a.axpby = 1;
+            a.alpha = epi.alpha;
+            a.beta = epi.beta;
+        }
         e = pade::launch_spike<T>(a, v->M, pw, kadj, mode, s);
         if (e != hipSuccess) return hip_fail(e, "spike_apply launch");
         return 0;
     }
-    T *work = nullptr;
+    // generic path: it keeps intermediates in its output, so an epilogue
+    // goes through a temporary
+    const size_t count = (size_t)a.npencils * n;
+    T *work = nullptr, *tmp = nullptr;
     if (kadj) {
-        e = hipMallocAsync((void **)&work, sizeof(T) * (size_t)a.npencils * n, s);
+        e = hipMallocAsync((void **)&work, sizeof(T) * count, s);
         if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(work)");
     }
+    if (epi.on) {
+        e = hipMallocAsync((void **)&tmp, sizeof(T) * count, s);
+        if (e != hipSuccess) {
+            if (work) (void)hipFreeAsync(work, s);
+            return hip_fail(e, "hipMallocAsync(tmp)");
+        }
+        a.out = tmp;
+    }
     e = pade::launch_generic<T>(a, kadj, work, s);
-    if (work) hipFreeAsync(work, s);
+    if (e == hipSuccess && epi.on) e = pade::launch_axpby<T>(tmp, df, epi.alpha, epi.beta, (long long)count, s);
+    if (work) (void)hipFreeAsync(work, s);
+    if (tmp) (void)hipFreeAsync(tmp, s);
     if (e != hipSuccess) return hip_fail(e, "generic_apply launch");
     return 0;
 }
 
 template <typename T>
 int apply_impl(pade_op_t op, int oper, int axis, const T *f, T *df, int na, int nb, int bc1, int bcn,
-               bool adj, hipStream_t s)
+               bool adj, hipStream_t s, Epilogue<T> epi)
 {
     if (!op) return fail(400, "null operator");
     if (!f || !df) return fail(400, "null array");
@@ -342,6 +370,11 @@ int apply_impl(pade_op_t op, int oper, int axis, const T *f, T *df, int na, int 
     }
     const long long count = (long long)na * nb * op->n;
     if (op->n == 1) {
+        if (epi.on) {   // op(f) is f (filter) or 0 (derivatives)
+            hipError_t e = oper == pade::FILTER ? pade::launch_axpby<T>(f, df, epi.alpha, epi.beta, count, s)
+                                                : pade::launch_axpby<T>(nullptr, df, T(0), epi.beta, count, s);
+            return e == hipSuccess ? 0 : hip_fail(e, "axpby");
+        }
         if (oper == pade::FILTER) {
             hipError_t e = hipMemcpyAsync(df, f, sizeof(T) * count, hipMemcpyDeviceToDevice, s);
             return e == hipSuccess ? 0 : hip_fail(e, "copy");
@@ -354,7 +387,7 @@ int apply_impl(pade_op_t op, int oper, int axis, const T *f, T *df, int na, int 
         hipError_t e = hipMallocAsync((void **)&tmp, sizeof(T) * count, s);
         if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(tmp)");
         int rc = apply_impl<T>(op, pade::D1, axis, f, tmp, na, nb, bc1, bcn, adj, s);
-        if (!rc) rc = apply_impl<T>(op, pade::D1, axis, tmp, df, na, nb, bc1, bcn, adj, s);
+        if (!rc) rc = apply_impl<T>(op, pade::D1, axis, tmp, df, na, nb, bc1, bcn, adj, s, epi);
         hipFreeAsync(tmp, s);
         return rc;
     }
@@ -363,7 +396,7 @@ int apply_impl(pade_op_t op, int oper, int axis, const T *f, T *df, int na, int 
     auto it = op->plans.find({oper, k1, kn, adj ? 1 : 0});
     if (it == op->plans.end())
         return fail(21, adj ? "adjoint needs n >= 16" : "no plan for this operation");
-    return apply_one<T>(op, it->second, axis, f, df, na, nb, adj, s);
+    return apply_one<T>(op, it->second, axis, f, df, na, nb, adj, s, epi);
 }
 
 }  // namespace
@@ -437,6 +470,26 @@ int pade_op_apply_adjoint_f32(pade_op_t op, int oper, int axis, const float *f, 
                               int nb, int bc1, int bcn, void *stream)
 {
     return apply_impl<float>(op, oper, axis, f, df, na, nb, bc1, bcn, true, (hipStream_t)stream);
+}
+
+int pade_op_apply_axpby(pade_op_t op, int oper, int axis, double alpha, const double *f, double beta, double *df,
+                        int na, int nb, int bc1, int bcn, void *stream)
+{
+    Epilogue<double> epi;
+    epi.on = true;
+    epi.alpha = alpha;
+    epi.beta = beta;
+    return apply_impl<double>(op, oper, axis, f, df, na, nb, bc1, bcn, false, (hipStream_t)stream, epi);
+}
+
+int pade_op_apply_axpby_f32(pade_op_t op, int oper, int axis, float alpha, const float *f, float beta, float *df,
+                            int na, int nb, int bc1, int bcn, void *stream)
+{
+    Epilogue<float> epi;
+    epi.on = true;
+    epi.alpha = alpha;
+    epi.beta = beta;
+    return apply_impl<float>(op, oper, axis, f, df, na, nb, bc1, bcn, false, (hipStream_t)stream, epi);
 }
 
 int pade_op_apply_host(pade_op_t op, int oper, int axis, const double *f, double *df, int na, int nb,

@@ -39,6 +39,8 @@ struct ApplyArgs {
     int ncls;              // number of coefficient blocks
     int kq;                // reduced-system window: partitions p-kq..p+kq (2kq+1 >= P: all)
     int cls[kMaxPartitions];
+    int axpby;             // 0: out = op(f); 1: out = alpha * op(f) + beta * out
+    T alpha, beta;
 };
 
 size_t spike_lds_bytes(size_t tsize, int M, int P, int PW, bool adj, int mode, int coef_len);
@@ -48,5 +50,9 @@ hipError_t launch_spike(const ApplyArgs<T> &a, int M, int PW, bool adj, int mode
 
 template <typename T>
 hipError_t launch_generic(const ApplyArgs<T> &a, bool adj, T *work, hipStream_t s);
+
+// y = alpha * x + beta * y over `count` elements (x == nullptr: y = beta * y).
+template <typename T>
+hipError_t launch_axpby(const T *x, T *y, T alpha, T beta, long long count, hipStream_t s);
 
 }  // namespace pade

@@ -33,6 +33,7 @@
 // cover (n not a multiple of 16, or more than 16 partitions).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "pade_kernels.h"
@@ -102,7 +103,7 @@ struct XTile {
 
 // LDS layout (elements of T):
 //   tips  [4P][NPEN]
-//   halo  [8P][NPEN]               adjoint only
+//   halo  [8P][NPEN]               partition end values (halo exchange)
 //   bnd   [256]                    boundary rows (left 128, right 128)
 //   coef  [coef_len]               coefficient classes + reduced-system rows
 //   xt    [waves][64][PAD]         x pencils only
@@ -110,7 +111,7 @@ __host__ __device__ inline int lds_elems(int M, int P, int PW, bool adj, int mod
 {
     const int npen = 64 / PW;
     const int waves = (P + PW - 1) / PW;
-    int e = (adj ? 12 : 4) * P * npen + 256 + coef_len;
+    int e = 12 * P * npen + 256 + coef_len;
     if (mode == 1) e += waves * 64 * 9;
     (void)M;
     return e;
@@ -210,8 +211,8 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
     const bool edge_right = BND && p == P - 1;
 
     T *tips = smem;                                     // [4P][NPEN]
-    T *halo = smem + 4 * P * NPEN;                      // [8P][NPEN]  (adjoint)
-    T *bnd = smem + (ADJ ? 12 : 4) * P * NPEN;          // [256]
+    T *halo = smem + 4 * P * NPEN;                      // [8P][NPEN]
+    T *bnd = smem + 12 * P * NPEN;                      // [256]
     T *coef = bnd + 256;                                // [coef_len]
     T *xt = coef + a.coef_len + wave * 64 * XTile<PW>::PAD;
 
@@ -233,29 +234,21 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
         if constexpr (!ADJ) {
 #pragma unroll
             for (int t = 4; t < M + 4; t++) w[t] = ld(fb + (long long)(pw0 - 4 + t) * stp, active ? hoff : loff);
-            // halo rows (wrapped per partition): full per-lane addresses
-            const T *fq = a.f + base0 + (long long)(loff / sizeof(T));
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int jl = wrap_index(pM - 4 + k, n, !BND);
-                const int jr = wrap_index(pM + M + k, n, !BND);
-                w[k] = fq[(long long)jl * stp];
-                w[M + 4 + k] = fq[(long long)jr * stp];
-            }
         } else {
 #pragma unroll
             for (int t = 0; t < M; t++) w[t] = ld(fb + (long long)(pw0 + t) * stp, active ? hoff : loff);
         }
     } else {
         // x pencils: tile rows r = (pencil, half) of this wave, XT points per
-        // chunk.  The forward window [-4, M+4) of each partition (halo
-        // included, wrapped or clamped per element) is (M+8)/XT coalesced
-        // chunks; the adjoint needs only [0, M).
+        // chunk; each chunk is XT contiguous points of one row, aligned to
+        // XT * sizeof(T) bytes, so every load instruction covers whole
+        // aligned segments.  Only the partition's own M points are loaded;
+        // the forward halo comes from the neighbouring partitions via LDS.
         constexpr int XT = XTile<PW>::XT, PAD = XTile<PW>::PAD;
         constexpr int G = 64 / XT;                       // rows per load instruction
-        constexpr int NCH = ADJ ? M / XT : (M + 8) / XT;
-        constexpr int P0 = ADJ ? 0 : -4;                 // first window point
-        static_assert((M + 8) % XT == 0 && M % XT == 0, "chunking");
+        constexpr int NCH = M / XT;
+        constexpr int W0 = ADJ ? 0 : 4;                  // register slot of the first point
+        static_assert(M % XT == 0, "chunking");
 #pragma unroll
         for (int c = 0; c < NCH; c++) {
 #pragma unroll
@@ -264,20 +257,44 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
                 const int rp = r / PW, rh = r % PW;
                 const int part = wave * PW + rh;
                 const bool ok = q0 + rp < a.npencils && part < P;
-                const int j = wrap_index((ok ? part : 0) * M + P0 + c * XT + (lane % XT), n, !BND);
+                const int j = (ok ? part : 0) * M + c * XT + (lane % XT);
                 const unsigned xo = ok ? (unsigned)((rp * (long long)n + j) * (long long)sizeof(T)) : 0u;
-                w[c * XT + s] = ld(fb, xo);
+                w[W0 + c * XT + s] = ld(fb, xo);
             }
         }
         const int myrow = pen * PW + half;
 #pragma unroll
         for (int c = 0; c < NCH; c++) {
 #pragma unroll
-            for (int s = 0; s < XT; s++) xt[(s * G + lane / XT) * PAD + (lane % XT)] = w[c * XT + s];
+            for (int s = 0; s < XT; s++) xt[(s * G + lane / XT) * PAD + (lane % XT)] = w[W0 + c * XT + s];
             wave_lds_sync();
 #pragma unroll
-            for (int j = 0; j < XT; j++) w[c * XT + j] = xt[myrow * PAD + j];
+            for (int j = 0; j < XT; j++) w[W0 + c * XT + j] = xt[myrow * PAD + j];
             wave_lds_sync();
+        }
+    }
+
+    if constexpr (!ADJ) {
+        // ------------------------- forward halo: neighbouring partitions' ends
+        // Periodic pencils wrap; at the ends of a bounded pencil the window is
+        // padded with the end value (those stencil rows are replaced by the
+        // boundary rows).
+        if (active) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                halo[(8 * p + k) * NPEN + pen] = w[4 + k];
+                halo[(8 * p + 4 + k) * NPEN + pen] = w[M + k];
+            }
+        }
+        __syncthreads();
+        int pl = pc - 1, pr = pc + 1;
+        bool hasl = true, hasr = true;
+        if (pl < 0) { if (!BND) pl += P; else hasl = false; }
+        if (pr >= P) { if (!BND) pr -= P; else hasr = false; }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            w[k] = hasl ? halo[(8 * pl + 4 + k) * NPEN + pen] : w[4];
+            w[M + 4 + k] = hasr ? halo[(8 * pr + k) * NPEN + pen] : w[M + 3];
         }
     }
 
@@ -384,8 +401,16 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
     if constexpr (MODE == 0) {
         const unsigned hoff = loff + (unsigned)((long long)half * M * stp * (long long)sizeof(T));
         if (valid) {
+            if (a.axpby) {
 #pragma unroll
-            for (int t = 0; t < M; t++) st(ob + (long long)(pw0 + t) * stp, hoff, w[t]);
+                for (int t = 0; t < M; t++) {
+                    const T old = ld(ob + (long long)(pw0 + t) * stp, hoff);
+                    st(ob + (long long)(pw0 + t) * stp, hoff, a.alpha * w[t] + a.beta * old);
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < M; t++) st(ob + (long long)(pw0 + t) * stp, hoff, w[t]);
+            }
         }
     } else {
         constexpr int XT = XTile<PW>::XT, PAD = XTile<PW>::PAD;
@@ -409,7 +434,9 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
                 const long long qq = q0 + rp;
                 if (qq < a.npencils && wave * PW + rh < P) {
                     const unsigned xo = (unsigned)((rp * (long long)n + rh * M + (lane % XT)) * (long long)sizeof(T));
-                    st(ob + pw0 + c * XT, xo, w[c * XT + s]);
+                    T v = w[c * XT + s];
+                    if (a.axpby) v = a.alpha * v + a.beta * ld(ob + pw0 + c * XT, xo);
+                    st(ob + pw0 + c * XT, xo, v);
                 }
             }
         }
@@ -555,9 +582,29 @@ hipError_t launch_generic(const ApplyArgs<T> &a, bool adj, T *work, hipStream_t 
     return hipGetLastError();
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void axpby_kernel(const T *__restrict__ x, T *__restrict__ y, T alpha, T beta,
+                                                   long long count)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
+        y[i] = (x ? alpha * x[i] : T(0)) + beta * y[i];
+}
+
+template <typename T>
+hipError_t launch_axpby(const T *x, T *y, T alpha, T beta, long long count, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    const long long blocks = std::min<long long>((count + 255) / 256, 65536);
+    hipLaunchKernelGGL((axpby_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, s, x, y, alpha, beta, count);
+    return hipGetLastError();
+}
+
 template hipError_t launch_spike<double>(const ApplyArgs<double> &, int, int, bool, int, hipStream_t);
 template hipError_t launch_spike<float>(const ApplyArgs<float> &, int, int, bool, int, hipStream_t);
 template hipError_t launch_generic<double>(const ApplyArgs<double> &, bool, double *, hipStream_t);
 template hipError_t launch_generic<float>(const ApplyArgs<float> &, bool, float *, hipStream_t);
+template hipError_t launch_axpby<double>(const double *, double *, double, double, long long, hipStream_t);
+template hipError_t launch_axpby<float>(const float *, float *, float, float, long long, hipStream_t);
 
 }  // namespace pade

@@ -111,7 +111,9 @@ class _CompactOp:
         return self._h
 
     # -- application -----------------------------------------------------------
-    def _apply(self, oper, axis, f, df, na, nb, bc1=None, bcn=None, adjoint=False):
+    def _apply(self, oper, axis, f, df, na, nb, bc1=None, bcn=None, adjoint=False, alpha=None, beta=None):
+        """df = op(f) (adjoint: op^T f); with alpha / beta given,
+        df = alpha * op(f) + beta * df in the same pass (device arrays)."""
         if not self._h:
             raise PadeError(400, "operator used before init")
         b1 = 0 if bc1 is None else int(bc1)
@@ -133,6 +135,15 @@ class _CompactOp:
                                               df.ctypes.data_as(dp), int(na), int(nb), b1, bn))
             return df
         _check_device_arrays(f, df, count)
+        if alpha is not None or beta is not None:
+            if adjoint:
+                raise TypeError("the accumulate form is provided for the forward operator")
+            a = 1.0 if alpha is None else float(alpha)
+            b = 0.0 if beta is None else float(beta)
+            fn = lib.pade_op_apply_axpby if f.dtype == torch.float64 else lib.pade_op_apply_axpby_f32
+            _lib.check(fn(self._h, oper, axis, a, ctypes.c_void_p(f.data_ptr()), b, ctypes.c_void_p(df.data_ptr()),
+                          int(na), int(nb), b1, bn, _stream_ptr()))
+            return df
         if f.dtype == torch.float64:
             fn = lib.pade_op_apply_adjoint if adjoint else lib.pade_op_apply
         else:

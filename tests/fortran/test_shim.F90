@@ -72,7 +72,7 @@ program test_shim
     p1 = c06%cd06der2(f(:, 3, 5))
     g = 0.0_rk
     g(:, 1, 1) = p1
-    call put("cd06_p_der2_pencil", g)
+    call put("cd06_p_der2", g)
     call c06%destroy()
 
     ierr = c90%init(n1, .false.)

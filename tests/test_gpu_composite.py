@@ -1,0 +1,142 @@
+"""The accumulate epilogue (df = alpha * op(f) + beta * df) and the composite
+operators gradient / curl / divergence / filter3D / crossprod
+(src/utilities/operators.F90:18-243) against the oracle."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import pade_dense as pd
+import padeops_amd as po
+from padeops_amd import composite
+from padeops_amd._lib import PADE_D1, PADE_D2, PADE_FILTER
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+OPER = {"d1": PADE_D1, "d2": PADE_D2, "filter": PADE_FILTER}
+
+
+def t(a, dtype=torch.float64):
+    return po.as_fortran(torch.tensor(np.asarray(a), device=DEV, dtype=dtype))
+
+
+def relerr(a, b):
+    a = a.double().cpu().numpy() if torch.is_tensor(a) else a
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def ref(kind, which, f, axis, periodic, bc=(0, 0)):
+    n = f.shape[axis]
+    dx = 2 * math.pi / n if periodic else 1.0 / (n - 1)
+    return pd.apply_axis(pd.dense_operator(kind, which, n, dx, periodic, bc[0], bc[1]), f, axis)
+
+
+@pytest.mark.parametrize("kind,which,periodic,bc", [("cd10", "d1", True, (0, 0)), ("cd10", "d2", False, (1, 0)),
+                                                    ("cf90", "filter", False, (0, -1)), ("cd06", "d1", False, (0, 0)),
+                                                    ("cd06", "d2", False, (0, 0))])
+@pytest.mark.parametrize("n", [64, 33, 1])
+@pytest.mark.parametrize("axis", [1, 2, 3])
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 3e-4)])
+def test_accumulate_epilogue(kind, which, periodic, bc, n, axis, dtype, tol):
+    if n == 1 and not periodic and kind != "cf90":
+        periodic = True
+    if n < 10 and kind == "cf90" and n != 1:
+        pytest.skip("cf90 needs n >= 10")
+    rng = np.random.default_rng(n + axis)
+    shape = [9, 7]
+    shape.insert(axis - 1, n)
+    f = rng.standard_normal(shape)
+    d0 = rng.standard_normal(shape)
+    dx = 2 * math.pi / n if periodic else 1.0 / max(n - 1, 1)
+    op = {"cd06": po.cd06, "cd10": po.cd10, "cf90": po.cf90}[kind]()
+    assert (op.init(n, periodic) if kind == "cf90" else op.init(n, dx, periodic, bc[0], bc[1])) == 0
+    ft, dft = t(f, dtype), t(d0, dtype)
+    na, nb = [s for i, s in enumerate(shape) if i != axis - 1]
+    op._apply(OPER[which], axis, ft, dft, na, nb, bc[0], bc[1], alpha=-1.5, beta=0.75)
+    torch.cuda.synchronize()
+    if n == 1:
+        opf = f if which == "filter" else np.zeros_like(f)
+    else:
+        opf = ref(kind, which, f, axis - 1, periodic, (0, 0) if kind == "cd06" else bc)
+    expect = -1.5 * opf + 0.75 * d0
+    assert relerr(dft, expect) < tol
+
+
+def facade(shape):
+    nx, ny, nz = shape
+    d = po.derivatives()
+    d.init(nx, ny, nz, 2 * math.pi / nx, 1.0 / (ny - 1), 1.0 / (nz - 1), True, False, False, "cd10", "cd06", "cd10")
+    return d
+
+
+def ref_d1(f, axis, bc):
+    # x: cd10 periodic; y: cd06 bounded (bc ignored); z: cd10 bounded with bc
+    if axis == 0:
+        return ref("cd10", "d1", f, 0, True)
+    if axis == 1:
+        return ref("cd06", "d1", f, 1, False)
+    return ref("cd10", "d1", f, 2, False, bc)
+
+
+def fields(shape, k, seed):
+    rng = np.random.default_rng(seed)
+    return [rng.standard_normal(shape) for _ in range(k)]
+
+
+def test_gradient_divergence_curl():
+    shape = (32, 24, 20)
+    d = facade(shape)
+    u, v, w = fields(shape, 3, 3)
+    zb = (1, -1)
+    U, V, W = t(u), t(v), t(w)
+    g = [po.fortran_empty(shape, device=DEV) for _ in range(3)]
+    composite.gradient(None, d, U, *g, z_bc=zb)
+    torch.cuda.synchronize()
+    for i in range(3):
+        assert relerr(g[i], ref_d1(u, i, zb)) < 1e-12
+    div = po.fortran_empty(shape, device=DEV)
+    composite.divergence(None, d, U, V, W, div, z_bc=zb)
+    torch.cuda.synchronize()
+    nz = (-zb[0], -zb[1])
+    expect = ref_d1(u, 0, nz) + ref_d1(v, 1, nz) + ref_d1(w, 2, nz)
+    assert relerr(div, expect) < 1e-12
+    c = [po.fortran_empty(shape, device=DEV) for _ in range(3)]
+    composite.curl(None, d, U, V, W, c, z_bc=zb)
+    torch.cuda.synchronize()
+    ex = [ref_d1(w, 1, zb) - ref_d1(v, 2, zb), ref_d1(u, 2, zb) - ref_d1(w, 0, zb),
+          ref_d1(v, 0, zb) - ref_d1(u, 1, zb)]
+    for i in range(3):
+        assert relerr(c[i], ex[i]) < 1e-12
+
+
+@pytest.mark.parametrize("numtimes", [1, 2])
+def test_filter3d(numtimes):
+    shape = (32, 24, 20)
+    fil = po.filters()
+    fil.init(*shape, True, False, False, "cf90", "cf90", "cf90")
+    (a,) = fields(shape, 1, 8)
+    arr = t(a)
+    composite.filter3D(None, fil, arr, numtimes=numtimes, z_bc=(0, 1))
+    torch.cuda.synchronize()
+    e = a
+    for _ in range(numtimes):
+        e = ref("cf90", "filter", e, 1, False)
+    for _ in range(numtimes):
+        e = ref("cf90", "filter", e, 0, True)
+    for _ in range(numtimes):
+        e = ref("cf90", "filter", e, 2, False, (0, 1))
+    assert relerr(arr, e) < 1e-12
+
+
+def test_crossprod():
+    shape = (8, 6, 4)
+    u = [t(a) for a in fields(shape, 3, 1)]
+    v = [t(a) for a in fields(shape, 3, 2)]
+    r = [po.fortran_empty(shape, device=DEV) for _ in range(3)]
+    composite.crossprod(r, u, v)
+    uu = np.stack([x.cpu().numpy() for x in u], -1)
+    vv = np.stack([x.cpu().numpy() for x in v], -1)
+    ex = np.cross(uu, vv)
+    for i in range(3):
+        assert relerr(r[i], ex[..., i]) < 1e-14

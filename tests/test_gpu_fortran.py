@@ -87,7 +87,7 @@ def test_fortran_cd06der2_pencil(records):
     n = f.shape[0]
     D = pd.dense_operator("cd06", "d2", n, 2 * math.pi / n, True, 0, 0)
     expect = D @ f
-    got = records["cd06_p_der2_pencil"][:, 0, 0]
+    got = records["cd06_p_der2"][:, 0, 0]
     assert float(np.abs(got - expect).max() / np.abs(expect).max()) < 1e-12
 
 

@@ -192,9 +192,40 @@ def main():
             table[f"{kind}_{which}_{'periodic' if periodic else 'bounded'}" + ("" if bc is None else f"_bc{bc[0]}{bc[1]}")] = row
             o.destroy()
         extra["fp64_gpts_by_operator"] = table
+        # composite operator: divergence with the sums fused into the producing
+        # kernels (64 B/pt) against three applies plus two array adds (96 B/pt)
+        log("bench: divergence")
+        from padeops_amd import composite
+        der = po.derivatives()
+        der.init(n, n, n, dx, dx, dx, True, True, True, "cd10", "cd10", "cd10")
+        v = torch.empty_like(f).copy_(f)
+        w = torch.empty_like(f).copy_(f)
+        div = po.fortran_empty((n, n, n), device=dev)
+        tmp = po.fortran_empty((n, n, n), device=dev)
+
+        def unfused():
+            der.ddy(v, div)
+            der.ddx(f, tmp)
+            div.add_(tmp)
+            der.ddz(w, tmp)
+            div.add_(tmp)
+
+        ms_fused = timed(lambda: composite.divergence(None, der, f, v, w, div), max(3, args.steps // 5), 2)
+        ms_unf = timed(unfused, max(3, args.steps // 5), 2)
+        extra["divergence"] = {"fused_gpts": pts / (ms_fused * 1e-3) / 1e9,
+                               "unfused_gpts": pts / (ms_unf * 1e-3) / 1e9,
+                               "fused_hbm_frac": 64.0 * pts / (ms_fused * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        del v, w, div, tmp
+        der.destroy()
         # practical ceiling on this machine: a plain device copy of the same field
         ms_copy = timed(lambda: df.copy_(f), args.steps, args.warmup)
         extra["device_copy_gbs"] = 16.0 * pts / (ms_copy * 1e-3) / 1e9
+        import ctypes
+        lib = _lib.load()
+        sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        ms_pc = timed(lambda: lib.pade_pencil_copy(ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(df.data_ptr()),
+                                                   8 * pts, sp), args.steps, args.warmup)
+        extra["pencil_copy_gbs"] = 16.0 * pts / (ms_pc * 1e-3) / 1e9
         log("bench: FP32 banded")
         f32 = po.fortran_empty((n, n, n), dtype=torch.float32, device=dev)
         f32.copy_(f)

@@ -571,7 +571,7 @@ int pade_pencil_copy(const void *src, void *dst, long long nbytes, void *stream)
 {
     if (!src || !dst) return fail(400, "null array");
     if (nbytes < 0) return fail(400, "negative size");
-    hipError_t e = hipMemcpyAsync(dst, src, (size_t)nbytes, hipMemcpyDeviceToDevice, (hipStream_t)stream);
+    hipError_t e = pade::pencil_copy_launch(src, dst, nbytes, (hipStream_t)stream);
     return e == hipSuccess ? 0 : hip_fail(e, "pencil copy");
 }
 

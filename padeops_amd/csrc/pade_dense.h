@@ -50,6 +50,8 @@ void dense_release(DenseCache &cache);
 hipError_t dense_launch(const DenseForm &D, int axis, const void *f, int f_dtype, void *df, int df_dtype,
                         int na, int nb, int prec, hipStream_t s);
 
+hipError_t pencil_copy_launch(const void *src, void *dst, long long nbytes, hipStream_t s);
+
 hipError_t permute_launch(const void *src, void *dst, int elem_size, int n1, int n2, int n3, int perm,
                           hipStream_t s);
 

@@ -373,13 +373,17 @@ __device__ __forceinline__ void make_chunk(const TI (&raw)[8], RingChunk<PREC> &
 
 constexpr int kRingSlots = 8;
 
-template <int PREC, typename TI, typename TO, bool CONTIG>
+// The D slots are re-read from LDS for every block, which keeps two waves per
+// SIMD (holding all 8 slots in registers measured slower at one wave per
+// SIMD).  PF: blocks of input prefetched ahead.
+template <int PREC, typename TI, typename TO, bool CONTIG, int PF>
 __global__ __launch_bounds__(256) void dense_ring(DenseArgs a)
 {
     constexpr int S = kRingSlots;
+    constexpr int NA = PREC == kDenseBF16x3 ? 3 : 2;     // 16-byte A words per slot and lane
     extern __shared__ __align__(16) unsigned char smem_raw[];
     u32x4 *tl = reinterpret_cast<u32x4 *>(smem_raw);
-    constexpr int tile_u4 = PREC == kDenseBF16x3 ? 3 * 64 : 2 * 64;
+    constexpr int tile_u4 = NA * 64;
     {
         const u32x4 *src = reinterpret_cast<const u32x4 *>(a.tiles);
         for (int i = threadIdx.x; i < S * tile_u4; i += 256) tl[i] = src[i];
@@ -411,41 +415,41 @@ __global__ __launch_bounds__(256) void dense_ring(DenseArgs a)
 #pragma unroll
         for (int j = 0; j < S; j++) make_chunk<PREC, TI>(raw[j], W[j]);
     }
-    // Raw chunks entering the window of the next block (nx1) and of the one
-    // after (nx2): two blocks of MFMA work cover each load's latency.  The
-    // chunk index wraps, so the prefetches past the last block stay in bounds.
-    TI nx1[2][8], nx2[2][8];
+    // nxq[k]: the two raw chunks entering the window of block ib + 1 + k.
+    // The chunk index wraps, so the prefetches past the last block stay in
+    // bounds.
+    TI nxq[PF][2][8];
 #pragma unroll
-    for (int j = 0; j < 2; j++) load8<TI, CONTIG>(f, chunk_off(2 - a.c_off + S - 2 + j), a.stride, nx1[j]);
+    for (int k = 0; k < PF - 1; k++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            load8<TI, CONTIG>(f, chunk_off(2 * (1 + k) - a.c_off + S - 2 + j), a.stride, nxq[k][j]);
     for (int ib = 0; ib < nblk; ib++) {
 #pragma unroll
         for (int j = 0; j < 2; j++)
-            load8<TI, CONTIG>(f, chunk_off(2 * (ib + 2) - a.c_off + S - 2 + j), a.stride, nx2[j]);
-        // The D tiles are re-read from LDS for every block (an opaque zero
-        // offset keeps the compiler from hoisting all 8 slots into registers,
-        // which would halve the waves per SIMD).
+            load8<TI, CONTIG>(f, chunk_off(2 * (ib + PF) - a.c_off + S - 2 + j), a.stride, nxq[PF - 1][j]);
+        // an opaque zero offset keeps the compiler from hoisting all 8 slots
+        // of D into registers (which would halve the waves per SIMD)
         int lds_off = 0;
         asm volatile("" : "+v"(lds_off));
         const u32x4 *tb = tl + lds_off;
-        f32x16 hi = f32x16{}, lo = f32x16{};
+        f32x16 hi = f32x16{}, lo = f32x16{}, lo2 = f32x16{};
 #pragma unroll
         for (int s = 0; s < S; s++) {
+            u32x4 A[NA];
+#pragma unroll
+            for (int k = 0; k < NA; k++) A[k] = tb[(s * NA + k) * 64 + lane];
             if constexpr (PREC == kDenseBF16x3) {
-                const u32x4 a0 = tb[(s * 3 + 0) * 64 + lane];
-                const u32x4 a1 = tb[(s * 3 + 1) * 64 + lane];
-                const u32x4 a2 = tb[(s * 3 + 2) * 64 + lane];
-                hi = mfma_bf16(a0, W[s].l0, hi);
-                lo = mfma_bf16(a0, W[s].l1, lo);
-                lo = mfma_bf16(a1, W[s].l0, lo);
-                lo = mfma_bf16(a0, W[s].l2, lo);
-                lo = mfma_bf16(a1, W[s].l1, lo);
-                lo = mfma_bf16(a2, W[s].l0, lo);
+                hi = mfma_bf16(A[0], W[s].l0, hi);
+                lo = mfma_bf16(A[0], W[s].l1, lo);
+                lo = mfma_bf16(A[1], W[s].l0, lo);
+                lo2 = mfma_bf16(A[0], W[s].l2, lo2);
+                lo2 = mfma_bf16(A[1], W[s].l1, lo2);
+                lo2 = mfma_bf16(A[2], W[s].l0, lo2);
             } else {
-                const u32x4 a0 = tb[(s * 2 + 0) * 64 + lane];
-                const u32x4 a1 = tb[(s * 2 + 1) * 64 + lane];
-                const float av[8] = {__uint_as_float(a0[0]), __uint_as_float(a0[1]), __uint_as_float(a0[2]),
-                                     __uint_as_float(a0[3]), __uint_as_float(a1[0]), __uint_as_float(a1[1]),
-                                     __uint_as_float(a1[2]), __uint_as_float(a1[3])};
+                const float av[8] = {__uint_as_float(A[0][0]), __uint_as_float(A[0][1]), __uint_as_float(A[0][2]),
+                                     __uint_as_float(A[0][3]), __uint_as_float(A[1][0]), __uint_as_float(A[1][1]),
+                                     __uint_as_float(A[1][2]), __uint_as_float(A[1][3])};
 #pragma unroll
                 for (int e = 0; e < 8; e += 2) {
                     hi = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e], W[s].v[e], hi, 0, 0, 0);
@@ -457,19 +461,22 @@ __global__ __launch_bounds__(256) void dense_ring(DenseArgs a)
 #pragma unroll
             for (int qd = 0; qd < 4; qd++) {
                 const long long row = 32LL * ib + 8 * qd + 4 * g;
-                store4<TO, CONTIG>(o, base + row * a.stride, a.stride, hi[4 * qd] + lo[4 * qd],
-                                   hi[4 * qd + 1] + lo[4 * qd + 1], hi[4 * qd + 2] + lo[4 * qd + 2],
-                                   hi[4 * qd + 3] + lo[4 * qd + 3]);
+                float v[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) v[e] = hi[4 * qd + e] + (lo[4 * qd + e] + lo2[4 * qd + e]);
+                store4<TO, CONTIG>(o, base + row * a.stride, a.stride, v[0], v[1], v[2], v[3]);
             }
         }
 #pragma unroll
         for (int j = 0; j < S - 2; j++) W[j] = W[j + 2];
-        make_chunk<PREC, TI>(nx1[0], W[S - 2]);
-        make_chunk<PREC, TI>(nx1[1], W[S - 1]);
+        make_chunk<PREC, TI>(nxq[0][0], W[S - 2]);
+        make_chunk<PREC, TI>(nxq[0][1], W[S - 1]);
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+        for (int k = 0; k < PF - 1; k++)
 #pragma unroll
-            for (int e = 0; e < 8; e++) nx1[j][e] = nx2[j][e];
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int e = 0; e < 8; e++) nxq[k][j][e] = nxq[k + 1][j][e];
     }
 }
 
@@ -479,7 +486,7 @@ hipError_t dense_launch_ring(const DenseForm &D, const DenseArgs &a, bool contig
     const size_t lds = (size_t)D.S * (PREC == kDenseBF16x3 ? 3072 : 2048);
     const long long nblk = (a.npencils + 127) / 128;
     if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
-    auto kern = contig ? dense_ring<PREC, TI, TO, true> : dense_ring<PREC, TI, TO, false>;
+    auto kern = contig ? dense_ring<PREC, TI, TO, true, 2> : dense_ring<PREC, TI, TO, false, 2>;
     hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(256), lds, s, a);
     return hipGetLastError();
 }
@@ -622,6 +629,47 @@ static hipError_t permute_t(const T *src, T *dst, long long n1, long long n2, lo
     hipLaunchKernelGGL((permute_tile<T>), g, dim3(256), 0, s, src, dst, n1, n3, n2, n1 * n2, n1,
                        n3 * n2, n3);
     return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// Pencil copy (one-rank re-orientation): a streaming copy with 16-byte lanes,
+// four loads in flight per thread before the matching stores.
+__global__ __launch_bounds__(256) void stream_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                    long long n16)
+{
+    constexpr int U = 4;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x; b < n16; b += stride * U) {
+        uint4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const long long i = b + u * stride;
+            if (i < n16) v[u] = src[i];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const long long i = b + u * stride;
+            if (i < n16) dst[i] = v[u];
+        }
+    }
+}
+
+hipError_t pencil_copy_launch(const void *src, void *dst, long long nbytes, hipStream_t s)
+{
+    if (nbytes <= 0) return hipSuccess;
+    const bool aligned = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15u) == 0;
+    if (!aligned || nbytes < (1 << 20))
+        return hipMemcpyAsync(dst, src, (size_t)nbytes, hipMemcpyDeviceToDevice, s);
+    const long long n16 = nbytes / 16;
+    const long long blocks = std::min<long long>((n16 + 1023) / 1024, 16384);
+    hipLaunchKernelGGL(stream_copy16, dim3((unsigned)blocks), dim3(256), 0, s, reinterpret_cast<const uint4 *>(src),
+                       reinterpret_cast<uint4 *>(dst), n16);
+    hipError_t e = hipGetLastError();
+    const long long tail = nbytes - n16 * 16;
+    if (e == hipSuccess && tail > 0)
+        e = hipMemcpyAsync(static_cast<char *>(dst) + n16 * 16, static_cast<const char *>(src) + n16 * 16,
+                           (size_t)tail, hipMemcpyDeviceToDevice, s);
+    return e;
 }
 
 hipError_t permute_launch(const void *src, void *dst, int elem_size, int n1, int n2, int n3, int perm,

@@ -94,32 +94,34 @@ __device__ __forceinline__ void pin(T &x) { asm volatile("" : "+v"(x)); }
         if (((t) % kRegion) == kRegion - 1) mem_region_fence(); \
     } while (0)
 
-// x-pencil transpose chunk: points per partition per LDS round trip
-template <int PW>
-struct XTile {
-    static constexpr int XT = 8;
-    static constexpr int PAD = XT + 1;
-};
+// x-pencil transpose tile: XT points of a row per chunk (aligned segments
+// of 64 bytes for FP64, 128 bytes for FP32 -- the widths that keep the FP64
+// M = 64 kernel within 256 registers and the FP32 one within 128); the halo
+// chunk holds the 4 + 4 window points either side of the partition.
+__host__ __device__ constexpr int xtile_points(int tsize, int M)
+{
+    return (tsize == 8 ? 8 : 32) < M ? (tsize == 8 ? 8 : 32) : M;
+}
+constexpr int kHaloChunk = 8;
 
 // LDS layout (elements of T):
 //   tips  [4P][NPEN]
-//   halo  [8P][NPEN]               partition end values (halo exchange)
+//   halo  [8P][NPEN]               adjoint only (partition end values)
 //   bnd   [256]                    boundary rows (left 128, right 128)
 //   coef  [coef_len]               coefficient classes + reduced-system rows
-//   xt    [waves][64][PAD]         x pencils only
-__host__ __device__ inline int lds_elems(int M, int P, int PW, bool adj, int mode, int coef_len)
+//   xt    [waves][64][XT + 1]      x pencils only
+__host__ __device__ inline int lds_elems(int M, int P, int PW, bool adj, int mode, int coef_len, int tsize)
 {
     const int npen = 64 / PW;
     const int waves = (P + PW - 1) / PW;
-    int e = 12 * P * npen + 256 + coef_len;
-    if (mode == 1) e += waves * 64 * 9;
-    (void)M;
+    int e = (adj ? 12 : 4) * P * npen + 256 + coef_len;
+    if (mode == 1) e += waves * 64 * (xtile_points(tsize, M) + 1);
     return e;
 }
 
 size_t spike_lds_bytes(size_t tsize, int M, int P, int PW, bool adj, int mode, int coef_len)
 {
-    return (size_t)lds_elems(M, P, PW, adj, mode, coef_len) * tsize;
+    return (size_t)lds_elems(M, P, PW, adj, mode, coef_len, (int)tsize) * tsize;
 }
 
 template <typename T, int M>
@@ -179,7 +181,7 @@ constexpr int spike_waves_per_eu()
     return sizeof(T) == 8 ? (M >= 64 ? 2 : 4) : (M >= 64 ? 4 : (M >= 32 ? 4 : 8));
 }
 
-template <typename T, int M, int PW, bool ADJ, int MODE, bool BND>
+template <typename T, int M, int PW, bool ADJ, int MODE, bool BND, bool EPI>
 __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
 {
     constexpr int NPEN = 64 / PW;
@@ -211,12 +213,15 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
     const bool edge_right = BND && p == P - 1;
 
     T *tips = smem;                                     // [4P][NPEN]
-    T *halo = smem + 4 * P * NPEN;                      // [8P][NPEN]
-    T *bnd = smem + 12 * P * NPEN;                      // [256]
+    T *halo = smem + 4 * P * NPEN;                      // [8P][NPEN]  (adjoint)
+    T *bnd = smem + (ADJ ? 12 : 4) * P * NPEN;          // [256]
     T *coef = bnd + 256;                                // [coef_len]
-    T *xt = coef + a.coef_len + wave * 64 * XTile<PW>::PAD;
+    constexpr int XT = xtile_points(sizeof(T), M), PAD = XT + 1;
+    T *xt = coef + a.coef_len + wave * 64 * PAD;
 
-    // --------------------------- 0. coefficients -> LDS (ahead of the field)
+    // ----------------------------------------------- 0. coefficients -> LDS
+    // (broadcast LDS reads measured faster than scalar loads of the same
+    // wave-uniform coefficients)
     {
         const T *g = tab + kOffCoef;
         const int len = a.coef_len;
@@ -228,27 +233,51 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
     T w[M + 8];
 
     // --------------------------------------------------------------- 1. load
+    T hv[kHaloChunk];                                    // x halo chunk (forward)
     if constexpr (MODE == 0) {
         // main rows: uniform base of this wave's first partition + per-lane offset
         const unsigned hoff = loff + (unsigned)((long long)half * M * stp * (long long)sizeof(T));
         if constexpr (!ADJ) {
 #pragma unroll
             for (int t = 4; t < M + 4; t++) w[t] = ld(fb + (long long)(pw0 - 4 + t) * stp, active ? hoff : loff);
+            // halo rows (wrapped per partition, or clamped at the ends of a
+            // bounded pencil, where the boundary rows replace the stencil)
+            const T *fq = a.f + base0 + (long long)(loff / sizeof(T));
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int jl = wrap_index(pM - 4 + k, n, !BND);
+                const int jr = wrap_index(pM + M + k, n, !BND);
+                w[k] = fq[(long long)jl * stp];
+                w[M + 4 + k] = fq[(long long)jr * stp];
+            }
         } else {
 #pragma unroll
             for (int t = 0; t < M; t++) w[t] = ld(fb + (long long)(pw0 + t) * stp, active ? hoff : loff);
         }
     } else {
-        // x pencils: tile rows r = (pencil, half) of this wave, XT points per
-        // chunk; each chunk is XT contiguous points of one row, aligned to
-        // XT * sizeof(T) bytes, so every load instruction covers whole
-        // aligned segments.  Only the partition's own M points are loaded;
-        // the forward halo comes from the neighbouring partitions via LDS.
-        constexpr int XT = XTile<PW>::XT, PAD = XTile<PW>::PAD;
+        // x pencils: tile rows r = (pencil, half) of this wave.  Main chunks:
+        // XT contiguous points of one row per lane group, aligned to whole
+        // cache lines; the forward halo chunk: 4 + 4 points around the
+        // partition (wrapped, or clamped at the ends of a bounded pencil).
         constexpr int G = 64 / XT;                       // rows per load instruction
         constexpr int NCH = M / XT;
         constexpr int W0 = ADJ ? 0 : 4;                  // register slot of the first point
         static_assert(M % XT == 0, "chunking");
+        if constexpr (!ADJ) {
+            constexpr int GH = 64 / kHaloChunk;
+#pragma unroll
+            for (int s = 0; s < kHaloChunk; s++) {
+                const int r = s * GH + lane / kHaloChunk;
+                const int rp = r / PW, rh = r % PW;
+                const int part = wave * PW + rh;
+                const bool ok = q0 + rp < a.npencils && part < P;
+                const int k = lane % kHaloChunk;
+                const int j0 = (ok ? part : 0) * M + (k < 4 ? k - 4 : M + k - 4);
+                const int j = wrap_index(j0, n, !BND);
+                const unsigned xo = ok ? (unsigned)((rp * (long long)n + j) * (long long)sizeof(T)) : 0u;
+                hv[s] = ld(fb, xo);
+            }
+        }
 #pragma unroll
         for (int c = 0; c < NCH; c++) {
 #pragma unroll
@@ -262,7 +291,27 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
                 w[W0 + c * XT + s] = ld(fb, xo);
             }
         }
+    }
+
+    if constexpr (MODE == 1) {
+        // wave-local transposes: row-major tile -> one row per lane
+        constexpr int G = 64 / XT;
+        constexpr int NCH = M / XT;
+        constexpr int W0 = ADJ ? 0 : 4;
         const int myrow = pen * PW + half;
+        if constexpr (!ADJ) {
+            constexpr int GH = 64 / kHaloChunk;
+#pragma unroll
+            for (int s = 0; s < kHaloChunk; s++)
+                xt[(s * GH + lane / kHaloChunk) * PAD + (lane % kHaloChunk)] = hv[s];
+            wave_lds_sync();
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                w[k] = xt[myrow * PAD + k];
+                w[M + 4 + k] = xt[myrow * PAD + 4 + k];
+            }
+            wave_lds_sync();
+        }
 #pragma unroll
         for (int c = 0; c < NCH; c++) {
 #pragma unroll
@@ -271,30 +320,6 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
 #pragma unroll
             for (int j = 0; j < XT; j++) w[W0 + c * XT + j] = xt[myrow * PAD + j];
             wave_lds_sync();
-        }
-    }
-
-    if constexpr (!ADJ) {
-        // ------------------------- forward halo: neighbouring partitions' ends
-        // Periodic pencils wrap; at the ends of a bounded pencil the window is
-        // padded with the end value (those stencil rows are replaced by the
-        // boundary rows).
-        if (active) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                halo[(8 * p + k) * NPEN + pen] = w[4 + k];
-                halo[(8 * p + 4 + k) * NPEN + pen] = w[M + k];
-            }
-        }
-        __syncthreads();
-        int pl = pc - 1, pr = pc + 1;
-        bool hasl = true, hasr = true;
-        if (pl < 0) { if (!BND) pl += P; else hasl = false; }
-        if (pr >= P) { if (!BND) pr -= P; else hasr = false; }
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            w[k] = hasl ? halo[(8 * pl + 4 + k) * NPEN + pen] : w[4];
-            w[M + 4 + k] = hasr ? halo[(8 * pr + k) * NPEN + pen] : w[M + 3];
         }
     }
 
@@ -401,7 +426,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
     if constexpr (MODE == 0) {
         const unsigned hoff = loff + (unsigned)((long long)half * M * stp * (long long)sizeof(T));
         if (valid) {
-            if (a.axpby) {
+            if constexpr (EPI) {
 #pragma unroll
                 for (int t = 0; t < M; t++) {
                     const T old = ld(ob + (long long)(pw0 + t) * stp, hoff);
@@ -413,7 +438,6 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
             }
         }
     } else {
-        constexpr int XT = XTile<PW>::XT, PAD = XTile<PW>::PAD;
         constexpr int G = 64 / XT;
         const int myrow = pen * PW + half;
 #pragma unroll
@@ -425,18 +449,29 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
             for (int s = 0; s < XT; s++) w[c * XT + s] = xt[(s * G + lane / XT) * PAD + (lane % XT)];
             wave_lds_sync();
         }
+        // every row of this wave's tile is stored (the common case): no
+        // per-store conditions
+        const bool full_tile = q0 + NPEN <= a.npencils && (wave + 1) * PW <= P;
+        auto store_row = [&](int c, int s) {
+            const int r = s * G + lane / XT;
+            const int rp = r / PW, rh = r % PW;
+            const unsigned xo = (unsigned)((rp * (long long)n + rh * M + (lane % XT)) * (long long)sizeof(T));
+            T v = w[c * XT + s];
+            if constexpr (EPI) v = a.alpha * v + a.beta * ld(ob + pw0 + c * XT, xo);
+            st(ob + pw0 + c * XT, xo, v);
+        };
+        if (full_tile) {
 #pragma unroll
-        for (int c = 0; c < M / XT; c++) {
+            for (int c = 0; c < M / XT; c++)
 #pragma unroll
-            for (int s = 0; s < XT; s++) {
-                const int r = s * G + lane / XT;
-                const int rp = r / PW, rh = r % PW;
-                const long long qq = q0 + rp;
-                if (qq < a.npencils && wave * PW + rh < P) {
-                    const unsigned xo = (unsigned)((rp * (long long)n + rh * M + (lane % XT)) * (long long)sizeof(T));
-                    T v = w[c * XT + s];
-                    if (a.axpby) v = a.alpha * v + a.beta * ld(ob + pw0 + c * XT, xo);
-                    st(ob + pw0 + c * XT, xo, v);
+                for (int s = 0; s < XT; s++) store_row(c, s);
+        } else {
+#pragma unroll
+            for (int c = 0; c < M / XT; c++) {
+#pragma unroll
+                for (int s = 0; s < XT; s++) {
+                    const int r = s * G + lane / XT;
+                    if (q0 + r / PW < a.npencils && wave * PW + r % PW < P) store_row(c, s);
                 }
             }
         }
@@ -530,12 +565,24 @@ static hipError_t launch_spike_t(const ApplyArgs<T> &a, hipStream_t s)
     const long long nblk = (a.npencils + NPEN - 1) / NPEN;
     const int waves = (a.P + PW - 1) / PW;
     const size_t lds = spike_lds_bytes(sizeof(T), M, a.P, PW, ADJ, MODE, a.coef_len);
-    if (a.periodic)
-        hipLaunchKernelGGL((spike_apply<T, M, PW, ADJ, MODE, false>), dim3((unsigned)nblk), dim3(64 * waves), lds,
-                           s, a);
-    else
-        hipLaunchKernelGGL((spike_apply<T, M, PW, ADJ, MODE, true>), dim3((unsigned)nblk), dim3(64 * waves), lds,
-                           s, a);
+    // the accumulate epilogue exists for the forward operator only
+    if (ADJ && a.axpby) return hipErrorInvalidValue;
+    const bool epi = !ADJ && a.axpby;
+    if (a.periodic) {
+        if (epi)
+            hipLaunchKernelGGL((spike_apply<T, M, PW, ADJ, MODE, false, !ADJ>), dim3((unsigned)nblk),
+                               dim3(64 * waves), lds, s, a);
+        else
+            hipLaunchKernelGGL((spike_apply<T, M, PW, ADJ, MODE, false, false>), dim3((unsigned)nblk),
+                               dim3(64 * waves), lds, s, a);
+    } else {
+        if (epi)
+            hipLaunchKernelGGL((spike_apply<T, M, PW, ADJ, MODE, true, !ADJ>), dim3((unsigned)nblk),
+                               dim3(64 * waves), lds, s, a);
+        else
+            hipLaunchKernelGGL((spike_apply<T, M, PW, ADJ, MODE, true, false>), dim3((unsigned)nblk),
+                               dim3(64 * waves), lds, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -274,3 +274,17 @@ def test_dense_not_provided():
     with pytest.raises(po.PadeError) as e:
         op.dense_apply(_lib.PADE_D1, 1, f, torch.zeros_like(f), 4, 4, _lib.PADE_DENSE_F32)
     assert e.value.code == 21
+
+
+@pytest.mark.parametrize("nbytes,off", [(3 << 20, 0), ((5 << 20) + 7, 0), ((2 << 20) + 5, 8), (1000, 0), (0, 0)])
+def test_pencil_copy_bitwise(nbytes, off):
+    import ctypes
+
+    src = torch.randint(0, 256, (nbytes + 64,), dtype=torch.uint8, device=DEV)
+    dst = torch.zeros_like(src)
+    lib = _lib.load()
+    _lib.check(lib.pade_pencil_copy(ctypes.c_void_p(src.data_ptr() + off), ctypes.c_void_p(dst.data_ptr() + off),
+                                    nbytes, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    torch.cuda.synchronize()
+    assert torch.equal(dst[off:off + nbytes], src[off:off + nbytes])
+    assert int(dst[off + nbytes:].sum()) == 0 and int(dst[:off].sum()) == 0

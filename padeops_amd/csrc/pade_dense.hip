@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 #include "pade_dense.h"
@@ -150,30 +151,29 @@ struct DenseArgs {
     int n, S, c_off;
 };
 
-__device__ __forceinline__ unsigned bf16_bits(float x)  // x rounded to BF16, as FP32 bits
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+
+// Two values rounded to BF16 (round to nearest even, v_cvt_pk_bf16_f32),
+// packed as one 32-bit word (first value in the low half); ra / rb receive
+// the exact remainders x - bf16(x).
+__device__ __forceinline__ unsigned bf16_pair(float a, float b, float &ra, float &rb)
 {
-    unsigned u = __float_as_uint(x);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return u & 0xffff0000u;
+    const bf16x2_t h = __builtin_convertvector((f32x2_t){a, b}, bf16x2_t);
+    const unsigned u = __builtin_bit_cast(unsigned, h);
+    ra = a - __uint_as_float(u << 16);
+    rb = b - __uint_as_float(u & 0xffff0000u);
+    return u;
 }
 
 __device__ __forceinline__ void split3(const float (&x)[8], u32x4 &l0, u32x4 &l1, u32x4 &l2)
 {
-    unsigned h0[8], h1[8], h2[8];
-#pragma unroll
-    for (int e = 0; e < 8; e++) {
-        float r = x[e];
-        h0[e] = bf16_bits(r);
-        r -= __uint_as_float(h0[e]);
-        h1[e] = bf16_bits(r);
-        r -= __uint_as_float(h1[e]);
-        h2[e] = bf16_bits(r);
-    }
 #pragma unroll
     for (int w = 0; w < 4; w++) {
-        l0[w] = (h0[2 * w] >> 16) | h0[2 * w + 1];
-        l1[w] = (h1[2 * w] >> 16) | h1[2 * w + 1];
-        l2[w] = (h2[2 * w] >> 16) | h2[2 * w + 1];
+        float r0, r1, s0, s1, t0, t1;
+        l0[w] = bf16_pair(x[2 * w], x[2 * w + 1], r0, r1);
+        l1[w] = bf16_pair(r0, r1, s0, s1);
+        l2[w] = bf16_pair(s0, s1, t0, t1);
     }
 }
 
@@ -377,7 +377,7 @@ constexpr int kRingSlots = 8;
 // SIMD (holding all 8 slots in registers measured slower at one wave per
 // SIMD).  PF: blocks of input prefetched ahead.
 template <int PREC, typename TI, typename TO, bool CONTIG, int PF>
-__global__ __launch_bounds__(256) void dense_ring(DenseArgs a)
+__device__ __forceinline__ void dense_ring_body(const DenseArgs &a)
 {
     constexpr int S = kRingSlots;
     constexpr int NA = PREC == kDenseBF16x3 ? 3 : 2;     // 16-byte A words per slot and lane
@@ -424,60 +424,107 @@ __global__ __launch_bounds__(256) void dense_ring(DenseArgs a)
 #pragma unroll
         for (int j = 0; j < 2; j++)
             load8<TI, CONTIG>(f, chunk_off(2 * (1 + k) - a.c_off + S - 2 + j), a.stride, nxq[k][j]);
-    for (int ib = 0; ib < nblk; ib++) {
+
+    float tok = 0.0f;
+    // One output block.  PH = ib % 4: window position s of block ib lives
+    // in W[(2 PH + s) % 8], so advancing the window is a renaming -- the two
+    // chunks entering block ib + 1 overwrite the slots of the two leaving.
+    auto block = [&](int ib, auto ph) {
+        constexpr int PH = decltype(ph)::value;
 #pragma unroll
         for (int j = 0; j < 2; j++)
             load8<TI, CONTIG>(f, chunk_off(2 * (ib + PF) - a.c_off + S - 2 + j), a.stride, nxq[PF - 1][j]);
-        // an opaque zero offset keeps the compiler from hoisting all 8 slots
-        // of D into registers (which would halve the waves per SIMD)
+        // An opaque zero offset, tied to the previous block's result, keeps
+        // the compiler from hoisting the 8 slots of D above the previous
+        // block (which would double the live registers and halve the waves
+        // per SIMD).
         int lds_off = 0;
-        asm volatile("" : "+v"(lds_off));
+        asm volatile("" : "+v"(lds_off) : "v"(tok));
         const u32x4 *tb = tl + lds_off;
         f32x16 hi = f32x16{}, lo = f32x16{}, lo2 = f32x16{};
 #pragma unroll
         for (int s = 0; s < S; s++) {
+            const RingChunk<PREC> &Wc = W[(2 * PH + s) % S];
             u32x4 A[NA];
 #pragma unroll
             for (int k = 0; k < NA; k++) A[k] = tb[(s * NA + k) * 64 + lane];
             if constexpr (PREC == kDenseBF16x3) {
-                hi = mfma_bf16(A[0], W[s].l0, hi);
-                lo = mfma_bf16(A[0], W[s].l1, lo);
-                lo = mfma_bf16(A[1], W[s].l0, lo);
-                lo2 = mfma_bf16(A[0], W[s].l2, lo2);
-                lo2 = mfma_bf16(A[1], W[s].l1, lo2);
-                lo2 = mfma_bf16(A[2], W[s].l0, lo2);
+                hi = mfma_bf16(A[0], Wc.l0, hi);
+                lo = mfma_bf16(A[0], Wc.l1, lo);
+                lo = mfma_bf16(A[1], Wc.l0, lo);
+                lo2 = mfma_bf16(A[0], Wc.l2, lo2);
+                lo2 = mfma_bf16(A[1], Wc.l1, lo2);
+                lo2 = mfma_bf16(A[2], Wc.l0, lo2);
             } else {
                 const float av[8] = {__uint_as_float(A[0][0]), __uint_as_float(A[0][1]), __uint_as_float(A[0][2]),
                                      __uint_as_float(A[0][3]), __uint_as_float(A[1][0]), __uint_as_float(A[1][1]),
                                      __uint_as_float(A[1][2]), __uint_as_float(A[1][3])};
 #pragma unroll
                 for (int e = 0; e < 8; e += 2) {
-                    hi = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e], W[s].v[e], hi, 0, 0, 0);
-                    lo = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e + 1], W[s].v[e + 1], lo, 0, 0, 0);
+                    hi = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e], Wc.v[e], hi, 0, 0, 0);
+                    lo = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e + 1], Wc.v[e + 1], lo, 0, 0, 0);
                 }
             }
         }
+        float res[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) res[e] = hi[e] + (lo[e] + lo2[e]);
         if (valid) {
 #pragma unroll
             for (int qd = 0; qd < 4; qd++) {
                 const long long row = 32LL * ib + 8 * qd + 4 * g;
-                float v[4];
-#pragma unroll
-                for (int e = 0; e < 4; e++) v[e] = hi[4 * qd + e] + (lo[4 * qd + e] + lo2[4 * qd + e]);
-                store4<TO, CONTIG>(o, base + row * a.stride, a.stride, v[0], v[1], v[2], v[3]);
+                store4<TO, CONTIG>(o, base + row * a.stride, a.stride, res[4 * qd], res[4 * qd + 1],
+                                   res[4 * qd + 2], res[4 * qd + 3]);
             }
         }
+        // The entering chunks are split only after this block's products:
+        // hoisted above them, the new limbs would be live beside the old ones.
 #pragma unroll
-        for (int j = 0; j < S - 2; j++) W[j] = W[j + 2];
-        make_chunk<PREC, TI>(nxq[0][0], W[S - 2]);
-        make_chunk<PREC, TI>(nxq[0][1], W[S - 1]);
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) asm volatile("" : "+v"(nxq[0][j][e]) : "v"(res[0]));
+        tok = res[0];
+        make_chunk<PREC, TI>(nxq[0][0], W[(2 * PH) % S]);
+        make_chunk<PREC, TI>(nxq[0][1], W[(2 * PH + 1) % S]);
 #pragma unroll
         for (int k = 0; k < PF - 1; k++)
 #pragma unroll
             for (int j = 0; j < 2; j++)
 #pragma unroll
                 for (int e = 0; e < 8; e++) nxq[k][j][e] = nxq[k + 1][j][e];
+        // keep the blocks apart: interleaving two blocks' work doubles the
+        // live registers and halves the waves per SIMD
+        asm volatile("" ::: "memory");
+    };
+    using P0 = std::integral_constant<int, 0>;
+    using P1 = std::integral_constant<int, 1>;
+    using P2 = std::integral_constant<int, 2>;
+    using P3 = std::integral_constant<int, 3>;
+    int ib = 0;
+    for (; ib + 3 < nblk; ib += 4) {
+        block(ib, P0{});
+        block(ib + 1, P1{});
+        block(ib + 2, P2{});
+        block(ib + 3, P3{});
     }
+    if (ib < nblk) block(ib, P0{});
+    if (ib + 1 < nblk) block(ib + 1, P1{});
+    if (ib + 2 < nblk) block(ib + 2, P2{});
+}
+
+// Contiguous FP32 inputs: register budget of two waves per SIMD (the
+// compiler would otherwise settle on one); FP64 or strided inputs need more
+// registers and keep the default.
+template <int PREC, typename TI, typename TO, bool CONTIG, int PF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void dense_ring_w2(DenseArgs a)
+{
+    dense_ring_body<PREC, TI, TO, CONTIG, PF>(a);
+}
+
+template <int PREC, typename TI, typename TO, bool CONTIG, int PF>
+__global__ __launch_bounds__(256) void dense_ring(DenseArgs a)
+{
+    dense_ring_body<PREC, TI, TO, CONTIG, PF>(a);
 }
 
 template <int PREC, typename TI, typename TO>
@@ -487,6 +534,8 @@ hipError_t dense_launch_ring(const DenseForm &D, const DenseArgs &a, bool contig
     const long long nblk = (a.npencils + 127) / 128;
     if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
     auto kern = contig ? dense_ring<PREC, TI, TO, true, 2> : dense_ring<PREC, TI, TO, false, 2>;
+    if constexpr (sizeof(TI) == 4)
+        if (contig) kern = dense_ring_w2<PREC, TI, TO, true, 2>;
     hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(256), lds, s, a);
     return hipGetLastError();
 }

@@ -221,7 +221,8 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
 
     // ----------------------------------------------- 0. coefficients -> LDS
     // (broadcast LDS reads measured faster than scalar loads of the same
-    // wave-uniform coefficients)
+    // wave-uniform coefficients, and this order -- table copy, then the
+    // field loads -- faster than issuing the field loads first)
     {
         const T *g = tab + kOffCoef;
         const int len = a.coef_len;

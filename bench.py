@@ -116,8 +116,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="time only the headline kernel")
     ap.add_argument("--traffic-driver", action="store_true",
-                    help="profiling driver: 3 plain 1 GiB copies (counter calibration), then cd10 "
-                         "periodic dd1, dd2, dd3 three times each; prints nothing else")
+                    help="profiling driver: 3 device copies and 3 pencil copies of 1 GiB (counter "
+                         "calibration), then cd10 "
+                         "periodic dd1, dd2, dd3 (FP64), dd1 (FP32) and the BF16x3 dense form "
+                         "three times each")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build targets one GPU: the operator path is measured on one "
@@ -136,11 +138,23 @@ def main():
         g = torch.empty_like(f)
         for _ in range(3):
             g.copy_(f)
+        import ctypes
+        lib = _lib.load()
+        for _ in range(3):   # the library's own streaming copy, same bytes
+            _lib.check(lib.pade_pencil_copy(ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(g.data_ptr()),
+                                            8 * n ** 3, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
         op = po.cd10()
         assert op.init(n, 2 * math.pi / n, True) == 0
         for fn in (op.dd1, op.dd2, op.dd3):
             for _ in range(3):
                 fn(f, g, n, n)
+        f32 = po.fortran_empty((n, n, n), dtype=torch.float32, device="cuda:0")
+        f32.copy_(f)
+        g32 = torch.empty_like(f32)
+        for _ in range(3):
+            op.dd1(f32, g32, n, n)
+        for _ in range(3):
+            op.dense_apply(_lib.PADE_D1, 1, f32, g32, n, n, _lib.PADE_DENSE_BF16X3)
         torch.cuda.synchronize()
         print("traffic driver done", flush=True)
         return

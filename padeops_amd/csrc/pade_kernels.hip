@@ -100,9 +100,12 @@ __device__ __forceinline__ void pin(T &x) { asm volatile("" : "+v"(x)); }
 // chunk holds the 4 + 4 window points either side of the partition.
 __host__ __device__ constexpr int xtile_points(int tsize, int M)
 {
-    return (tsize == 8 ? 8 : 32) < M ? (tsize == 8 ? 8 : 32) : M;
+    return (tsize == 8 ? 16 : 32) < M ? (tsize == 8 ? 16 : 32) : M;
 }
 constexpr int kHaloChunk = 8;
+// 16-byte lanes: VL elements per lane, XT / VL lanes per tile row; tile rows
+// padded by one lane width (16-byte aligned)
+__host__ __device__ constexpr int xtile_row(int tsize, int M) { return xtile_points(tsize, M) + 16 / tsize; }
 
 // LDS layout (elements of T):
 //   tips  [4P][NPEN]
@@ -115,7 +118,11 @@ __host__ __device__ inline int lds_elems(int M, int P, int PW, bool adj, int mod
     const int npen = 64 / PW;
     const int waves = (P + PW - 1) / PW;
     int e = (adj ? 12 : 4) * P * npen + 256 + coef_len;
-    if (mode == 1) e += waves * 64 * (xtile_points(tsize, M) + 1);
+    if (mode == 1) {
+        const int vl = 16 / tsize;
+        e = (e + vl - 1) / vl * vl;           // 16-byte aligned tile
+        e += waves * 64 * xtile_row(tsize, M);
+    }
     return e;
 }
 
@@ -189,7 +196,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
     T *smem = reinterpret_cast<T *>(smem_raw);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int half = PW == 2 ? (lane >> 5) : 0;
+    const int half = PW == 1 ? 0 : lane / NPEN;  // this lane's partition within the wave
     const int pen = lane & (NPEN - 1);
     const int P = a.P;
     const int n = a.n;
@@ -216,8 +223,12 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
     T *halo = smem + 4 * P * NPEN;                      // [8P][NPEN]  (adjoint)
     T *bnd = smem + (ADJ ? 12 : 4) * P * NPEN;          // [256]
     T *coef = bnd + 256;                                // [coef_len]
-    constexpr int XT = xtile_points(sizeof(T), M), PAD = XT + 1;
-    T *xt = coef + a.coef_len + wave * 64 * PAD;
+    constexpr int XT = xtile_points(sizeof(T), M), PAD = xtile_row(sizeof(T), M);
+    constexpr int VL = 16 / sizeof(T);                  // elements per 16-byte lane
+    constexpr int LPR = XT / VL;                        // lanes per tile row
+    typedef T vec_t __attribute__((ext_vector_type(VL)));
+    const int xt0 = ((int)(coef - smem) + a.coef_len + VL - 1) / VL * VL;
+    T *xt = smem + xt0 + wave * 64 * PAD;
 
     // ----------------------------------------------- 0. coefficients -> LDS
     // (broadcast LDS reads measured faster than scalar loads of the same
@@ -257,10 +268,11 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
         }
     } else {
         // x pencils: tile rows r = (pencil, half) of this wave.  Main chunks:
-        // XT contiguous points of one row per lane group, aligned to whole
-        // cache lines; the forward halo chunk: 4 + 4 points around the
-        // partition (wrapped, or clamped at the ends of a bounded pencil).
-        constexpr int G = 64 / XT;                       // rows per load instruction
+        // XT contiguous points of one row, 16 bytes per lane, so that every
+        // load instruction covers G whole 128-byte rows; the forward halo
+        // chunk: 4 + 4 points around the partition (wrapped, or clamped at
+        // the ends of a bounded pencil).
+        constexpr int G = 64 / LPR;                      // rows per load instruction
         constexpr int NCH = M / XT;
         constexpr int W0 = ADJ ? 0 : 4;                  // register slot of the first point
         static_assert(M % XT == 0, "chunking");
@@ -282,21 +294,23 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
 #pragma unroll
         for (int c = 0; c < NCH; c++) {
 #pragma unroll
-            for (int s = 0; s < XT; s++) {
-                const int r = s * G + lane / XT;             // tile row
+            for (int s = 0; s < XT / VL; s++) {
+                const int r = s * G + lane / LPR;            // tile row
                 const int rp = r / PW, rh = r % PW;
                 const int part = wave * PW + rh;
                 const bool ok = q0 + rp < a.npencils && part < P;
-                const int j = (ok ? part : 0) * M + c * XT + (lane % XT);
+                const int j = (ok ? part : 0) * M + c * XT + VL * (lane % LPR);
                 const unsigned xo = ok ? (unsigned)((rp * (long long)n + j) * (long long)sizeof(T)) : 0u;
-                w[W0 + c * XT + s] = ld(fb, xo);
+                const vec_t v = ld(reinterpret_cast<const vec_t *>(fb), xo);
+#pragma unroll
+                for (int e = 0; e < VL; e++) w[W0 + c * XT + s * VL + e] = v[e];
             }
         }
     }
 
     if constexpr (MODE == 1) {
         // wave-local transposes: row-major tile -> one row per lane
-        constexpr int G = 64 / XT;
+        constexpr int G = 64 / LPR;
         constexpr int NCH = M / XT;
         constexpr int W0 = ADJ ? 0 : 4;
         const int myrow = pen * PW + half;
@@ -316,10 +330,19 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
 #pragma unroll
         for (int c = 0; c < NCH; c++) {
 #pragma unroll
-            for (int s = 0; s < XT; s++) xt[(s * G + lane / XT) * PAD + (lane % XT)] = w[W0 + c * XT + s];
+            for (int s = 0; s < XT / VL; s++) {
+                vec_t v;
+#pragma unroll
+                for (int e = 0; e < VL; e++) v[e] = w[W0 + c * XT + s * VL + e];
+                *reinterpret_cast<vec_t *>(xt + (s * G + lane / LPR) * PAD + VL * (lane % LPR)) = v;
+            }
             wave_lds_sync();
 #pragma unroll
-            for (int j = 0; j < XT; j++) w[W0 + c * XT + j] = xt[myrow * PAD + j];
+            for (int j = 0; j < XT / VL; j++) {
+                const vec_t v = *reinterpret_cast<const vec_t *>(xt + myrow * PAD + VL * j);
+#pragma unroll
+                for (int e = 0; e < VL; e++) w[W0 + c * XT + j * VL + e] = v[e];
+            }
             wave_lds_sync();
         }
     }
@@ -439,39 +462,51 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
             }
         }
     } else {
-        constexpr int G = 64 / XT;
+        constexpr int G = 64 / LPR;
         const int myrow = pen * PW + half;
 #pragma unroll
         for (int c = 0; c < M / XT; c++) {
 #pragma unroll
-            for (int j = 0; j < XT; j++) xt[myrow * PAD + j] = w[c * XT + j];
+            for (int j = 0; j < XT / VL; j++) {
+                vec_t v;
+#pragma unroll
+                for (int e = 0; e < VL; e++) v[e] = w[c * XT + j * VL + e];
+                *reinterpret_cast<vec_t *>(xt + myrow * PAD + VL * j) = v;
+            }
             wave_lds_sync();
 #pragma unroll
-            for (int s = 0; s < XT; s++) w[c * XT + s] = xt[(s * G + lane / XT) * PAD + (lane % XT)];
+            for (int s = 0; s < XT / VL; s++) {
+                const vec_t v = *reinterpret_cast<const vec_t *>(xt + (s * G + lane / LPR) * PAD + VL * (lane % LPR));
+#pragma unroll
+                for (int e = 0; e < VL; e++) w[c * XT + s * VL + e] = v[e];
+            }
             wave_lds_sync();
         }
         // every row of this wave's tile is stored (the common case): no
         // per-store conditions
         const bool full_tile = q0 + NPEN <= a.npencils && (wave + 1) * PW <= P;
         auto store_row = [&](int c, int s) {
-            const int r = s * G + lane / XT;
+            const int r = s * G + lane / LPR;
             const int rp = r / PW, rh = r % PW;
-            const unsigned xo = (unsigned)((rp * (long long)n + rh * M + (lane % XT)) * (long long)sizeof(T));
-            T v = w[c * XT + s];
-            if constexpr (EPI) v = a.alpha * v + a.beta * ld(ob + pw0 + c * XT, xo);
-            st(ob + pw0 + c * XT, xo, v);
+            const unsigned xo =
+                (unsigned)((rp * (long long)n + rh * M + VL * (lane % LPR)) * (long long)sizeof(T));
+            vec_t v;
+#pragma unroll
+            for (int e = 0; e < VL; e++) v[e] = w[c * XT + s * VL + e];
+            if constexpr (EPI) v = a.alpha * v + a.beta * ld(reinterpret_cast<const vec_t *>(ob + pw0 + c * XT), xo);
+            st(reinterpret_cast<vec_t *>(ob + pw0 + c * XT), xo, v);
         };
         if (full_tile) {
 #pragma unroll
             for (int c = 0; c < M / XT; c++)
 #pragma unroll
-                for (int s = 0; s < XT; s++) store_row(c, s);
+                for (int s = 0; s < XT / VL; s++) store_row(c, s);
         } else {
 #pragma unroll
             for (int c = 0; c < M / XT; c++) {
 #pragma unroll
-                for (int s = 0; s < XT; s++) {
-                    const int r = s * G + lane / XT;
+                for (int s = 0; s < XT / VL; s++) {
+                    const int r = s * G + lane / LPR;
                     if (q0 + r / PW < a.npencils && wave * PW + r % PW < P) store_row(c, s);
                 }
             }

@@ -234,12 +234,24 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
     // (broadcast LDS reads measured faster than scalar loads of the same
     // wave-uniform coefficients, and this order -- table copy, then the
     // field loads -- faster than issuing the field loads first)
+    // Batches of eight loads per thread before their LDS writes, so a table
+    // costs one memory round trip per 8 x blockDim elements (bounded tables,
+    // with per-partition reduced-system rows, run to a few thousand).
     {
         const T *g = tab + kOffCoef;
         const int len = a.coef_len;
-        for (int idx = threadIdx.x; idx < len; idx += blockDim.x) coef[idx] = g[idx];
+        const int nt = blockDim.x;
+        constexpr int U = 8;
+        for (int b0 = threadIdx.x; b0 < len; b0 += U * nt) {
+            T v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) v[u] = g[min(b0 + u * nt, len - 1)];   // unconditional loads
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (b0 + u * nt < len) coef[b0 + u * nt] = v[u];
+        }
         if (BND)
-            for (int idx = threadIdx.x; idx < 256; idx += blockDim.x) bnd[idx] = tab[kOffBl + idx];
+            for (int idx = threadIdx.x; idx < 256; idx += nt) bnd[idx] = tab[kOffBl + idx];
     }
 
     T w[M + 8];

@@ -238,8 +238,10 @@ int env_int(const char *name)
 
 // Choose (partition size M, partitions per wave PW) for this launch.
 // Two partitions per wave (two workgroups per CU) throughout; x pencils
-// (LDS-transposed, mode 1) prefer M = 64, strided y/z pencils M = 32 -- the
-// fastest variants measured at 512^3 for FP64 and FP32.  Small problems take
+// (LDS-transposed, mode 1) and bounded y/z pencils prefer M = 64, periodic
+// y/z pencils M = 32 -- the fastest variants measured at 512^3 (bounded z:
+// 279 vs 214 Gpts/s; the M = 32 bounded kernel runs at three waves per SIMD,
+// one 8-wave workgroup per CU).  Small problems take
 // the smallest M to raise the wave count.  PADE_SPIKE_M / PADE_SPIKE_PW
 // override the choice (tuning).
 const Variant *pick_variant(const Plan &pl, long long npencils, int mode, int *pw_out)
@@ -247,7 +249,7 @@ const Variant *pick_variant(const Plan &pl, long long npencils, int mode, int *p
     static const int envM = env_int("PADE_SPIKE_M"), envPW = env_int("PADE_SPIKE_PW");
     const Variant *best = nullptr;
     int best_pw = 1, best_score = -1;
-    const int pref_m = mode == 1 ? 64 : 32;
+    const int pref_m = (mode == 1 || !pl.mats.periodic) ? 64 : 32;
     for (const Variant &v : pl.spike)
         for (int pw : {1, 2}) {
             if ((v.P + pw - 1) / pw > 8) continue;

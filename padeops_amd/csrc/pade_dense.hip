@@ -681,24 +681,27 @@ static hipError_t permute_t(const T *src, T *dst, long long n1, long long n2, lo
 }
 
 // ------------------------------------------------------------------------
-// Pencil copy (one-rank re-orientation): a streaming copy with 16-byte lanes,
-// four loads in flight per thread before the matching stores.
-__global__ __launch_bounds__(256) void stream_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+// Pencil copy (one-rank re-orientation): a streaming copy with 16-byte lanes.
+// Each workgroup copies one contiguous 16 KiB block (four loads in flight per
+// thread before the matching stores), with non-temporal loads and stores:
+// the data passes through once.  On MI355X this form measured 6.2-6.4 TB/s
+// for 1 GiB, against 5.4-5.6 TB/s with cached accesses.
+__global__ __launch_bounds__(256) void stream_copy16(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst,
                                                     long long n16)
 {
     constexpr int U = 4;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x; b < n16; b += stride * U) {
-        uint4 v[U];
+    const long long stride = (long long)gridDim.x * 256 * U;
+    for (long long b = (long long)blockIdx.x * 256 * U + threadIdx.x; b < n16; b += stride) {
+        u32x4 v[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const long long i = b + u * stride;
-            if (i < n16) v[u] = src[i];
+            const long long i = b + u * 256;
+            if (i < n16) v[u] = __builtin_nontemporal_load(src + i);
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const long long i = b + u * stride;
-            if (i < n16) dst[i] = v[u];
+            const long long i = b + u * 256;
+            if (i < n16) __builtin_nontemporal_store(v[u], dst + i);
         }
     }
 }
@@ -710,9 +713,9 @@ hipError_t pencil_copy_launch(const void *src, void *dst, long long nbytes, hipS
     if (!aligned || nbytes < (1 << 20))
         return hipMemcpyAsync(dst, src, (size_t)nbytes, hipMemcpyDeviceToDevice, s);
     const long long n16 = nbytes / 16;
-    const long long blocks = std::min<long long>((n16 + 1023) / 1024, 16384);
-    hipLaunchKernelGGL(stream_copy16, dim3((unsigned)blocks), dim3(256), 0, s, reinterpret_cast<const uint4 *>(src),
-                       reinterpret_cast<uint4 *>(dst), n16);
+    const long long blocks = std::min<long long>((n16 + 1023) / 1024, 1LL << 20);
+    hipLaunchKernelGGL(stream_copy16, dim3((unsigned)blocks), dim3(256), 0, s, reinterpret_cast<const u32x4 *>(src),
+                       reinterpret_cast<u32x4 *>(dst), n16);
     hipError_t e = hipGetLastError();
     const long long tail = nbytes - n16 * 16;
     if (e == hipSuccess && tail > 0)

@@ -77,6 +77,30 @@ __device__ __forceinline__ void st(T *u, unsigned off, T v)
     *reinterpret_cast<T *>(reinterpret_cast<char *>(u) + off) = v;
 }
 
+// Streaming (non-temporal) forms for the field.  Every output is written
+// once, so the stores do not need to stay in L2: with the nt hint they stop
+// evicting the lines that neighbouring partitions still read as halos (x
+// reads fall from 1.09x to 1.05x of the field) and the solve runs 2-4 %
+// faster on every axis.  The loads stay cached: nt loads sped up y by 2 %
+// but slowed x and z, whose halo lines are read twice (x reads 1.05x ->
+// 1.17x).  (A plain 1 GiB device copy goes from 5.6 to 6.4 TB/s with both.)
+constexpr bool kNtLoad = false;    // main field loads
+constexpr bool kNtStore = true;    // output stores
+template <typename T>
+__device__ __forceinline__ T ld_s(const T *u, unsigned off)
+{
+    const T *p = reinterpret_cast<const T *>(reinterpret_cast<const char *>(u) + off);
+    if constexpr (kNtLoad) return __builtin_nontemporal_load(p);
+    return *p;
+}
+template <typename T>
+__device__ __forceinline__ void st_s(T *u, unsigned off, T v)
+{
+    T *p = reinterpret_cast<T *>(reinterpret_cast<char *>(u) + off);
+    if constexpr (kNtStore) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
 // Scheduling region length (rows): a compiler memory barrier every kRegion
 // rows of the unrolled sweeps keeps the LDS coefficient reads next to their
 // use; left alone the compiler issues all of them up front and spills them.
@@ -263,7 +287,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
         const unsigned hoff = loff + (unsigned)((long long)half * M * stp * (long long)sizeof(T));
         if constexpr (!ADJ) {
 #pragma unroll
-            for (int t = 4; t < M + 4; t++) w[t] = ld(fb + (long long)(pw0 - 4 + t) * stp, active ? hoff : loff);
+            for (int t = 4; t < M + 4; t++) w[t] = ld_s(fb + (long long)(pw0 - 4 + t) * stp, active ? hoff : loff);
             // halo rows (wrapped per partition, or clamped at the ends of a
             // bounded pencil, where the boundary rows replace the stencil)
             const T *fq = a.f + base0 + (long long)(loff / sizeof(T));
@@ -276,33 +300,24 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
             }
         } else {
 #pragma unroll
-            for (int t = 0; t < M; t++) w[t] = ld(fb + (long long)(pw0 + t) * stp, active ? hoff : loff);
+            for (int t = 0; t < M; t++) w[t] = ld_s(fb + (long long)(pw0 + t) * stp, active ? hoff : loff);
         }
     } else {
         // x pencils: tile rows r = (pencil, half) of this wave.  Main chunks:
         // XT contiguous points of one row, 16 bytes per lane, so that every
-        // load instruction covers G whole 128-byte rows; the forward halo
-        // chunk: 4 + 4 points around the partition (wrapped, or clamped at
-        // the ends of a bounded pencil).
+        // load instruction covers G whole 128-byte rows.  Then the forward
+        // halo chunk (issued last, so that its lines are mostly in flight or
+        // in L2 already from the neighbouring partitions' main chunks),
+        // wrapped, or clamped at the ends of a bounded pencil:
+        //   PW == 2: one side per tile row -- the left halo of the pair's
+        //     first partition and the right halo of its second (of the first
+        //     when the pair has no second); the boundary inside the pair
+        //     comes from the main chunks in LDS;
+        //   PW == 1: both sides, 4 + 4 points.
         constexpr int G = 64 / LPR;                      // rows per load instruction
         constexpr int NCH = M / XT;
         constexpr int W0 = ADJ ? 0 : 4;                  // register slot of the first point
         static_assert(M % XT == 0, "chunking");
-        if constexpr (!ADJ) {
-            constexpr int GH = 64 / kHaloChunk;
-#pragma unroll
-            for (int s = 0; s < kHaloChunk; s++) {
-                const int r = s * GH + lane / kHaloChunk;
-                const int rp = r / PW, rh = r % PW;
-                const int part = wave * PW + rh;
-                const bool ok = q0 + rp < a.npencils && part < P;
-                const int k = lane % kHaloChunk;
-                const int j0 = (ok ? part : 0) * M + (k < 4 ? k - 4 : M + k - 4);
-                const int j = wrap_index(j0, n, !BND);
-                const unsigned xo = ok ? (unsigned)((rp * (long long)n + j) * (long long)sizeof(T)) : 0u;
-                hv[s] = ld(fb, xo);
-            }
-        }
 #pragma unroll
         for (int c = 0; c < NCH; c++) {
 #pragma unroll
@@ -313,9 +328,32 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
                 const bool ok = q0 + rp < a.npencils && part < P;
                 const int j = (ok ? part : 0) * M + c * XT + VL * (lane % LPR);
                 const unsigned xo = ok ? (unsigned)((rp * (long long)n + j) * (long long)sizeof(T)) : 0u;
-                const vec_t v = ld(reinterpret_cast<const vec_t *>(fb), xo);
+                const vec_t v = ld_s(reinterpret_cast<const vec_t *>(fb), xo);
 #pragma unroll
                 for (int e = 0; e < VL; e++) w[W0 + c * XT + s * VL + e] = v[e];
+            }
+        }
+        if constexpr (!ADJ) {
+            constexpr int HC = PW == 2 ? 4 : kHaloChunk;     // halo values per tile row
+            constexpr int GH = 64 / HC;
+#pragma unroll
+            for (int s = 0; s < HC; s++) {
+                const int r = s * GH + lane / HC;
+                const int rp = r / PW, rh = r % PW;
+                const int part = wave * PW + rh;
+                const int k = lane % HC;
+                bool ok = q0 + rp < a.npencils;
+                int j0;
+                if constexpr (PW == 2) {
+                    const int tp = min(part, P - 1);
+                    j0 = rh == 0 ? tp * M - 4 + k : tp * M + M + k;
+                } else {
+                    ok = ok && part < P;
+                    j0 = (ok ? part : 0) * M + (k < 4 ? k - 4 : M + k - 4);
+                }
+                const int j = wrap_index(j0, n, !BND);
+                const unsigned xo = ok ? (unsigned)((rp * (long long)n + j) * (long long)sizeof(T)) : 0u;
+                hv[s] = ld(fb, xo);
             }
         }
     }
@@ -326,19 +364,6 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
         constexpr int NCH = M / XT;
         constexpr int W0 = ADJ ? 0 : 4;
         const int myrow = pen * PW + half;
-        if constexpr (!ADJ) {
-            constexpr int GH = 64 / kHaloChunk;
-#pragma unroll
-            for (int s = 0; s < kHaloChunk; s++)
-                xt[(s * GH + lane / kHaloChunk) * PAD + (lane % kHaloChunk)] = hv[s];
-            wave_lds_sync();
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                w[k] = xt[myrow * PAD + k];
-                w[M + 4 + k] = xt[myrow * PAD + 4 + k];
-            }
-            wave_lds_sync();
-        }
 #pragma unroll
         for (int c = 0; c < NCH; c++) {
 #pragma unroll
@@ -354,6 +379,54 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
                 const vec_t v = *reinterpret_cast<const vec_t *>(xt + myrow * PAD + VL * j);
 #pragma unroll
                 for (int e = 0; e < VL; e++) w[W0 + c * XT + j * VL + e] = v[e];
+            }
+            if constexpr (PW == 2 && !ADJ) {
+                // the pair's shared boundary: the first partition's right halo
+                // is the second's first four points, the second's left halo the
+                // first's last four
+                if (c == 0) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const T t = xt[(pen * 2 + 1) * PAD + k];
+                        if (half == 0) w[M + 4 + k] = t;
+                    }
+                }
+                if (c == NCH - 1) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const T t = xt[(pen * 2) * PAD + XT - 4 + k];
+                        if (half == 1) w[k] = t;
+                    }
+                }
+            }
+            wave_lds_sync();
+        }
+        if constexpr (!ADJ) {
+            constexpr int HC = PW == 2 ? 4 : kHaloChunk;
+            constexpr int GH = 64 / HC;
+#pragma unroll
+            for (int s = 0; s < HC; s++)
+                xt[(s * GH + lane / HC) * PAD + (lane % HC)] = hv[s];
+            wave_lds_sync();
+            if constexpr (PW == 2) {
+                const bool pair = wave * 2 + 1 < P;      // uniform: the wave's second partition exists
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const T own = xt[myrow * PAD + k];
+                    const T nxt = xt[(pen * 2 + 1) * PAD + k];
+                    if (half == 0) {
+                        w[k] = own;
+                        if (!pair) w[M + 4 + k] = nxt;
+                    } else {
+                        w[M + 4 + k] = own;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    w[k] = xt[myrow * PAD + k];
+                    w[M + 4 + k] = xt[myrow * PAD + 4 + k];
+                }
             }
             wave_lds_sync();
         }
@@ -466,11 +539,11 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
 #pragma unroll
                 for (int t = 0; t < M; t++) {
                     const T old = ld(ob + (long long)(pw0 + t) * stp, hoff);
-                    st(ob + (long long)(pw0 + t) * stp, hoff, a.alpha * w[t] + a.beta * old);
+                    st_s(ob + (long long)(pw0 + t) * stp, hoff, a.alpha * w[t] + a.beta * old);
                 }
             } else {
 #pragma unroll
-                for (int t = 0; t < M; t++) st(ob + (long long)(pw0 + t) * stp, hoff, w[t]);
+                for (int t = 0; t < M; t++) st_s(ob + (long long)(pw0 + t) * stp, hoff, w[t]);
             }
         }
     } else {
@@ -506,7 +579,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
 #pragma unroll
             for (int e = 0; e < VL; e++) v[e] = w[c * XT + s * VL + e];
             if constexpr (EPI) v = a.alpha * v + a.beta * ld(reinterpret_cast<const vec_t *>(ob + pw0 + c * XT), xo);
-            st(reinterpret_cast<vec_t *>(ob + pw0 + c * XT), xo, v);
+            st_s(reinterpret_cast<vec_t *>(ob + pw0 + c * XT), xo, v);
         };
         if (full_tile) {
 #pragma unroll

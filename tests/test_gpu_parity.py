@@ -89,6 +89,26 @@ def test_bounded_fp64(kind, which, bc, n, axis):
     assert relerr(out, ref) < TOL64, (kind, which, bc, n, axis, relerr(out, ref))
 
 
+@pytest.mark.parametrize("kind,which,periodic,bc", [("cd10", "d1", True, None), ("cd10", "d2", False, (1, -1)),
+                                                     ("cf90", "filter", True, None), ("cd06", "d1", False, None),
+                                                     ("cd10", "d1", False, (0, 0))])
+@pytest.mark.parametrize("n", [96, 192, 320])
+@pytest.mark.parametrize("axis", [1, 3])
+def test_throughput_variants(kind, which, periodic, bc, n, axis):
+    """65,536 pencils: enough work for the throughput variants (two
+    partitions per wave, 32- or 64-point partitions) rather than the
+    small-batch ones.  n = 96, 192 and 320 give 3, 3 and 5 partitions, so the
+    last wave of each pencil group has a first partition without a second."""
+    if axis == 3 and n == 320:
+        pytest.skip("z at n = 320 adds nothing over n = 192")
+    dx = 2 * np.pi / n if periodic else 1.0 / (n - 1)
+    f = field(shape_for(axis, n, 256, 256), seed=n + 10 * axis)
+    op = make(kind, n, dx, periodic)
+    out = run(op, kind, which, axis, f, bc=bc)
+    ref = ref_apply(kind, which, f, axis, dx, periodic, *(bc or (0, 0)))
+    assert relerr(out, ref) < TOL64, (kind, which, periodic, bc, n, axis, relerr(out, ref))
+
+
 @pytest.mark.parametrize("init_bc", [(0, 0), (1, 1), (0, 1)])
 @pytest.mark.parametrize("axis", [1, 2, 3])
 def test_cd06_bounded(init_bc, axis):

@@ -267,20 +267,39 @@ def main():
                 err = errors(d32.permute(2, 1, 0), ref_x)
                 full = bool(prec & F)
                 msd = timed(run, max(3, args.steps // (10 if full else 2)), 2)
-                window, halfwidth = op.dense_info(_lib.PADE_D1, prec)
+                window, halfwidth, products = op.dense_info(_lib.PADE_D1, prec)
                 bf = (prec & ~F) == _lib.PADE_DENSE_BF16X3
-                flop_pt = 2.0 * window * (6 if bf else 1)
+                flop_pt = 32.0 * products          # executed MFMA flops per point
                 peak = BF16_MFMA_TFLOPS if bf else FP32_MFMA_TFLOPS
                 gpts = pts / (msd * 1e-3) / 1e9
                 extra[name] = {"gpts": gpts, "ms": msd, "err_max_rms": list(err),
                                "accepted": err[0] <= 1e-6,
                                "window_points": window, "decay_halfwidth": halfwidth,
+                               "mfma_products_per_block": products,
                                "mfma_flop_per_pt": flop_pt,
                                "mfma_frac": gpts * flop_pt / 1e3 / peak,
                                "hbm_frac": 8.0 * gpts / HBM_PEAK_GBS}
                 del f32, d32
             except po.PadeError as e:
                 extra[name] = {"unavailable": str(e)}
+        # The precision study: every x-derivative path at 512^3 with its error
+        # against the FP64 operator; the fastest path within the 1e-6 gate.
+        gate = 1e-6
+        paths = {"fp64_banded": (value, extra["fp64_x_err"][0]),
+                 "fp32_banded": (extra["fp32_banded"]["gpts"], extra["fp32_banded"]["err_max_rms"][0])}
+        for name in ("dense_fp32", "dense_bf16x3", "dense_fp32_full", "dense_bf16x3_full"):
+            if "gpts" in extra[name]:
+                paths[name] = (extra[name]["gpts"], extra[name]["err_max_rms"][0])
+        within = {k: v for k, v in paths.items() if v[1] <= gate}
+        best = max(within, key=lambda k: within[k][0])
+        extra["precision_study"] = {
+            "gate_max_rel_err": gate,
+            "paths": {k: {"gpts": v[0], "max_rel_err": v[1], "within_gate": v[1] <= gate} for k, v in paths.items()},
+            "fastest_within_gate": best,
+            "dense_bf16x3_over_fp32_banded": paths["dense_bf16x3"][0] / paths["fp32_banded"][0]
+            if "dense_bf16x3" in paths else None,
+            "dense_bf16x3_over_fp64_banded": paths["dense_bf16x3"][0] / value if "dense_bf16x3" in paths else None,
+        }
         del ref_x
 
     cpu = None

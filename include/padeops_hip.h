@@ -106,11 +106,17 @@ int pade_op_apply_host(pade_op_t op, int oper, int axis, const double *f, double
 int pade_op_dense_apply(pade_op_t op, int oper, int axis, const void *f, int f_dtype, void *df,
                         int df_dtype, int na, int nb, int bc1, int bcn, int prec, void *stream);
 
-/* Shape of the dense form that pade_op_dense_apply runs for (oper, prec):
- * *window = input points per output block that the GEMM visits (16 * slots;
- * n for PADE_DENSE_FULL), *halfwidth = decay half-width kept.  Builds and
- * caches the form on first use.  No reference counterpart. */
-int pade_op_dense_info(pade_op_t op, int oper, int prec, int *window, int *halfwidth);
+/* Shape of the dense form that pade_op_dense_apply runs for (oper, prec) on
+ * `axis` with the given dtypes: *window = input points per output block that
+ * the GEMM visits (16 * slots; n for PADE_DENSE_FULL), *halfwidth = decay
+ * half-width kept, *products = 16-point MFMA products per output block and
+ * pencil tile (BF16x3: up to 6 per slot -- fewer where the outer slots of a
+ * decaying window drop limb products whose total stays within 2^-28 of a
+ * row's absolute sum; FP32: one per slot).  MFMA flops per point are
+ * 32 * products.  Builds and caches the form on first use.  No reference
+ * counterpart. */
+int pade_op_dense_info(pade_op_t op, int oper, int prec, int axis, int f_dtype, int df_dtype, int *window,
+                       int *halfwidth, int *products);
 
 /* Pencil re-orientation on one GPU.  2DECOMP's transpose_x_to_y / y_to_z /
  * z_to_y / y_to_x (2decomp_fft/src/transpose_x_to_y.f90:14-91 and siblings)

@@ -60,6 +60,8 @@ def load():
     lib.pade_op_apply_host.argtypes = [c_void_p, c_int, c_int, dp, dp, c_int, c_int, c_int, c_int]
     lib.pade_op_dense_apply.argtypes = [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_int,
                                         c_int, c_int, c_int, c_int, c_void_p]
+    lib.pade_op_dense_info.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_int),
+                                       ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
     lib.pade_pencil_copy.argtypes = [c_void_p, c_void_p, c_ll, c_void_p]
     lib.pade_permute3.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p]
     lib.pade_plan_export.argtypes = [c_int, c_int, c_int, c_double, c_int, c_int, c_int, c_int, c_int, dp,

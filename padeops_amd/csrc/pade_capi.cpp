@@ -552,9 +552,13 @@ int pade_op_dense_apply(pade_op_t op, int oper, int axis, const void *f, int f_d
     return 0;
 }
 
-int pade_op_dense_info(pade_op_t op, int oper, int prec, int *window, int *halfwidth)
+int pade_op_dense_info(pade_op_t op, int oper, int prec, int axis, int f_dtype, int df_dtype, int *window,
+                       int *halfwidth, int *products)
 {
-    if (!op || !window || !halfwidth) return fail(400, "null argument");
+    if (!op || !window || !halfwidth || !products) return fail(400, "null argument");
+    if (axis < 1 || axis > 3) return fail(400, "axis must be 1, 2 or 3");
+    if ((f_dtype != PADE_F64 && f_dtype != PADE_F32) || (df_dtype != PADE_F64 && df_dtype != PADE_F32))
+        return fail(400, "dtype must be PADE_F64 or PADE_F32");
     if (!oper_valid(op->kind, oper)) return fail(21, "operation not provided for this operator kind");
     if (!op->periodic) return fail(21, "the dense form is provided for periodic operators");
     if (op->n < 64 || op->n % 64 != 0) return fail(21, "the dense form needs n a multiple of 64");
@@ -566,6 +570,8 @@ int pade_op_dense_info(pade_op_t op, int oper, int prec, int *window, int *halfw
     if (rc) return fail(rc >= 1000 ? rc : 21, "dense operator construction failed");
     *window = 16 * D->S;
     *halfwidth = D->halfwidth;
+    const int base_prec = prec & ~PADE_DENSE_FULL;
+    *products = pade::dense_products(*D, axis, f_dtype, df_dtype, base_prec);
     return 0;
 }
 

@@ -29,6 +29,10 @@ struct DenseForm {
     int S = 0;           // window slots (multiple of kDenseGroup)
     int c_off = 0;       // first chunk of the window relative to 2b
     int halfwidth = 0;   // decay half-width kept (n/2 for the full form)
+    // BF16x3 products per window slot: 0 all six in every slot; 1 the decay
+    // pattern {1, 3, 6, 6, 6, 6, 3, 1} of an 8-slot window (the limb products
+    // dropped in the outer slots sum to at most 2^-28 of a row's absolute sum)
+    int pattern = 0;
     void *tiles_bf = nullptr;
     float *tiles_f32 = nullptr;
 };
@@ -49,6 +53,11 @@ void dense_release(DenseCache &cache);
 // f_dtype / df_dtype: 0 FP64, 1 FP32.  prec: kDenseF32 or kDenseBF16x3.
 hipError_t dense_launch(const DenseForm &D, int axis, const void *f, int f_dtype, void *df, int df_dtype,
                         int na, int nb, int prec, hipStream_t s);
+
+// 16-point MFMA products per output block and pencil tile that dense_launch
+// runs for this form on `axis` (dtype codes 0 FP64, 1 FP32; prec kDenseF32 or
+// kDenseBF16x3): MFMA flops per point are 32 times this.
+int dense_products(const DenseForm &D, int axis, int f_dtype, int df_dtype, int prec);
 
 hipError_t pencil_copy_launch(const void *src, void *dst, long long nbytes, hipStream_t s);
 

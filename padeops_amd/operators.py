@@ -175,13 +175,16 @@ class _CompactOp:
                                            int(prec), _stream_ptr()))
         return df
 
-    def dense_info(self, oper, prec):
-        """(window, halfwidth) of the dense form: input points visited per
-        output block and the decay half-width kept."""
+    def dense_info(self, oper, prec, axis=1, f_dtype=_lib.PADE_F32, df_dtype=_lib.PADE_F32):
+        """(window, halfwidth, products) of the dense form applied along
+        ``axis`` with these dtypes: input points visited per output block, the
+        decay half-width kept, and the 16-point MFMA products per output block
+        and pencil tile (MFMA flops per point = 32 * products)."""
         lib = _lib.load()
-        w, h = ctypes.c_int(), ctypes.c_int()
-        _lib.check(lib.pade_op_dense_info(self._h, int(oper), int(prec), ctypes.byref(w), ctypes.byref(h)))
-        return w.value, h.value
+        w, h, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(lib.pade_op_dense_info(self._h, int(oper), int(prec), int(axis), int(f_dtype), int(df_dtype),
+                                          ctypes.byref(w), ctypes.byref(h), ctypes.byref(p)))
+        return w.value, h.value, p.value
 
 
 class cd06(_CompactOp):

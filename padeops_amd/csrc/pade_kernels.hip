@@ -133,15 +133,20 @@ __host__ __device__ constexpr int xtile_row(int tsize, int M) { return xtile_poi
 
 // LDS layout (elements of T):
 //   tips  [4P][NPEN]
-//   halo  [8P][NPEN]               adjoint only (partition end values)
+//   halo  [8P][NPEN]               adjoint, y/z pencils (partition end values)
 //   bnd   [256]                    boundary rows (left 128, right 128)
 //   coef  [coef_len]               coefficient classes + reduced-system rows
-//   xt    [waves][64][XT + 1]      x pencils only
+//   xt    [waves][64][XT + VL]     x pencils only; the x adjoint's halo
+//                                  exchange uses the start of this area (the
+//                                  tiles are idle between the load and store
+//                                  transposes), which keeps the M = 64 FP64
+//                                  adjoint at two workgroups per CU
 __host__ __device__ inline int lds_elems(int M, int P, int PW, bool adj, int mode, int coef_len, int tsize)
 {
     const int npen = 64 / PW;
     const int waves = (P + PW - 1) / PW;
-    int e = (adj ? 12 : 4) * P * npen + 256 + coef_len;
+    // (x pencils exchange the adjoint halo through their transpose tiles)
+    int e = (adj && mode != 1 ? 12 : 4) * P * npen + 256 + coef_len;
     if (mode == 1) {
         const int vl = 16 / tsize;
         e = (e + vl - 1) / vl * vl;           // 16-byte aligned tile
@@ -244,8 +249,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
     const bool edge_right = BND && p == P - 1;
 
     T *tips = smem;                                     // [4P][NPEN]
-    T *halo = smem + 4 * P * NPEN;                      // [8P][NPEN]  (adjoint)
-    T *bnd = smem + (ADJ ? 12 : 4) * P * NPEN;          // [256]
+    T *bnd = smem + (ADJ && MODE != 1 ? 12 : 4) * P * NPEN;   // [256]
     T *coef = bnd + 256;                                // [coef_len]
     constexpr int XT = xtile_points(sizeof(T), M), PAD = xtile_row(sizeof(T), M);
     constexpr int VL = 16 / sizeof(T);                  // elements per 16-byte lane
@@ -253,6 +257,8 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
     typedef T vec_t __attribute__((ext_vector_type(VL)));
     const int xt0 = ((int)(coef - smem) + a.coef_len + VL - 1) / VL * VL;
     T *xt = smem + xt0 + wave * 64 * PAD;
+    // [8P][NPEN] adjoint halo: 8P * NPEN <= waves * 64 * PAD since PAD >= 8
+    T *halo = MODE == 1 ? smem + xt0 : smem + 4 * P * NPEN;
 
     // ----------------------------------------------- 0. coefficients -> LDS
     // (broadcast LDS reads measured faster than scalar loads of the same
@@ -528,6 +534,10 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
             w[k] = (exch && hasl) ? halo[(8 * pl + 4 + k) * NPEN + pen] : T(0);
             w[M + 4 + k] = (exch && hasr) ? halo[(8 * pr + k) * NPEN + pen] : T(0);
         }
+        // x: the halo lives in the transpose tiles, which the store phase
+        // rewrites -- every wave reads its halo first
+        if constexpr (MODE == 1)
+            if (exch) __syncthreads();
         apply_stencil<T, M>(w, tab, bnd, edge_left, edge_right, a.nbl, a.nbr);
     }
 

@@ -8,8 +8,9 @@
 // Fast path (spike_apply): a pencil of n = P*M points is split into P
 // partitions of M points.  A workgroup holds NPEN = 64/PW pencils and all P
 // partitions of them; each wave serves PW partitions (PW = 1: 64 pencils x 1
-// partition, PW = 2: 32 pencils x 2 partitions), one (pencil, partition) per
-// lane with the M values in registers.  Each lane
+// partition, PW = 2: 32 pencils x 2 partitions, PW = 8: 8 whole pencils of
+// up to 8 partitions, x pencils only), one (pencil, partition) per lane with
+// the M values in registers.  Each lane
 //   1. loads its window (M points + 4-point halos) and evaluates the stencil,
 //   2. solves its partition's band block (forward + backward sweep) with
 //      coefficients staged once per workgroup in LDS,
@@ -81,16 +82,17 @@ __device__ __forceinline__ void st(T *u, unsigned off, T v)
 // once, so the stores do not need to stay in L2: with the nt hint they stop
 // evicting the lines that neighbouring partitions still read as halos (x
 // reads fall from 1.09x to 1.05x of the field) and the solve runs 2-4 %
-// faster on every axis.  The loads stay cached: nt loads sped up y by 2 %
-// but slowed x and z, whose halo lines are read twice (x reads 1.05x ->
-// 1.17x).  (A plain 1 GiB device copy goes from 5.6 to 6.4 TB/s with both.)
-constexpr bool kNtLoad = false;    // main field loads
+// faster on every axis.  The loads stay cached where other partitions read
+// the same lines again as halos: nt loads slowed x (two partitions per wave)
+// and z, whose halo lines are then fetched twice (x reads 1.05x -> 1.17x).
+// Whole-pencil x waves read every line once, and stream their loads.  (A
+// plain 1 GiB device copy goes from 5.6 to 6.4 TB/s with both.)
 constexpr bool kNtStore = true;    // output stores
-template <typename T>
+template <bool NT, typename T>
 __device__ __forceinline__ T ld_s(const T *u, unsigned off)
 {
     const T *p = reinterpret_cast<const T *>(reinterpret_cast<const char *>(u) + off);
-    if constexpr (kNtLoad) return __builtin_nontemporal_load(p);
+    if constexpr (NT) return __builtin_nontemporal_load(p);
     return *p;
 }
 template <typename T>
@@ -293,7 +295,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
         const unsigned hoff = loff + (unsigned)((long long)half * M * stp * (long long)sizeof(T));
         if constexpr (!ADJ) {
 #pragma unroll
-            for (int t = 4; t < M + 4; t++) w[t] = ld_s(fb + (long long)(pw0 - 4 + t) * stp, active ? hoff : loff);
+            for (int t = 4; t < M + 4; t++) w[t] = ld_s<false>(fb + (long long)(pw0 - 4 + t) * stp, active ? hoff : loff);
             // halo rows (wrapped per partition, or clamped at the ends of a
             // bounded pencil, where the boundary rows replace the stencil)
             const T *fq = a.f + base0 + (long long)(loff / sizeof(T));
@@ -306,7 +308,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
             }
         } else {
 #pragma unroll
-            for (int t = 0; t < M; t++) w[t] = ld_s(fb + (long long)(pw0 + t) * stp, active ? hoff : loff);
+            for (int t = 0; t < M; t++) w[t] = ld_s<false>(fb + (long long)(pw0 + t) * stp, active ? hoff : loff);
         }
     } else {
         // x pencils: tile rows r = (pencil, half) of this wave.  Main chunks:
@@ -319,7 +321,9 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
         //     first partition and the right halo of its second (of the first
         //     when the pair has no second); the boundary inside the pair
         //     comes from the main chunks in LDS;
-        //   PW == 1: both sides, 4 + 4 points.
+        //   PW == 1: both sides, 4 + 4 points;
+        //   PW == 8 (whole pencils per wave, P <= 8): none -- every halo is
+        //     in the wave's own tile.
         constexpr int G = 64 / LPR;                      // rows per load instruction
         constexpr int NCH = M / XT;
         constexpr int W0 = ADJ ? 0 : 4;                  // register slot of the first point
@@ -334,12 +338,12 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
                 const bool ok = q0 + rp < a.npencils && part < P;
                 const int j = (ok ? part : 0) * M + c * XT + VL * (lane % LPR);
                 const unsigned xo = ok ? (unsigned)((rp * (long long)n + j) * (long long)sizeof(T)) : 0u;
-                const vec_t v = ld_s(reinterpret_cast<const vec_t *>(fb), xo);
+                const vec_t v = ld_s<(PW >= 4)>(reinterpret_cast<const vec_t *>(fb), xo);
 #pragma unroll
                 for (int e = 0; e < VL; e++) w[W0 + c * XT + s * VL + e] = v[e];
             }
         }
-        if constexpr (!ADJ) {
+        if constexpr (!ADJ && PW <= 2) {   // (whole-pencil waves need no halo loads)
             constexpr int HC = PW == 2 ? 4 : kHaloChunk;     // halo values per tile row
             constexpr int GH = 64 / HC;
 #pragma unroll
@@ -404,10 +408,25 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
                         if (half == 1) w[k] = t;
                     }
                 }
+            } else if constexpr (PW >= 4 && !ADJ) {
+                // whole pencils in the wave (P <= PW): every halo is the end of
+                // a neighbouring partition in the tile, across the periodic
+                // wrap too (at the ends of a bounded pencil the boundary rows
+                // replace the rows that would read the wrapped values)
+                if (c == 0) {
+                    const int hr = half + 1 < P ? half + 1 : 0;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) w[M + 4 + k] = xt[(pen * PW + hr) * PAD + k];
+                }
+                if (c == NCH - 1) {
+                    const int hl = half >= 1 ? half - 1 : P - 1;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) w[k] = xt[(pen * PW + hl) * PAD + XT - 4 + k];
+                }
             }
             wave_lds_sync();
         }
-        if constexpr (!ADJ) {
+        if constexpr (!ADJ && PW <= 2) {
             constexpr int HC = PW == 2 ? 4 : kHaloChunk;
             constexpr int GH = 64 / HC;
 #pragma unroll
@@ -742,6 +761,15 @@ hipError_t launch_spike(const ApplyArgs<T> &a, int M, int PW, bool adj, int mode
         case 16: return launch_spike_mp<T, 16, 2>(a, adj, mode, s);
         case 32: return launch_spike_mp<T, 32, 2>(a, adj, mode, s);
         case 64: return launch_spike_mp<T, 64, 2>(a, adj, mode, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    // whole pencils per wave: x pencils only (strided rows of 8 pencils
+    // would split every load into 64-byte pieces), P <= 8
+    if (PW == 8 && mode == 1 && a.P <= 8) {
+        switch (M) {
+        case 32: return adj ? launch_spike_t<T, 32, 8, true, 1>(a, s) : launch_spike_t<T, 32, 8, false, 1>(a, s);
+        case 64: return adj ? launch_spike_t<T, 64, 8, true, 1>(a, s) : launch_spike_t<T, 64, 8, false, 1>(a, s);
         default: return hipErrorInvalidValue;
         }
     }

@@ -282,6 +282,22 @@ def test_dense_mixed_dtypes_and_ragged(axis):
             assert err < DENSE_TOL, (prec, in_dt, out_dt, err)
 
 
+def test_dense_info_products():
+    """The executed MFMA products per block: the decay pattern {1,3,6,6,6,6,3,1}
+    applies to cd10 d1 on x pencils with FP32 storage (the line ring); other
+    axes, FP64 storage and the FP32 products run the full window."""
+    n = 512
+    op = make("cd10", n, 2 * np.pi / n, True)
+    B, F = _lib.PADE_DENSE_BF16X3, _lib.PADE_DENSE_F32
+    assert op.dense_info(_lib.PADE_D1, B, 1, _lib.PADE_F32, _lib.PADE_F32) == (128, 36, 32)
+    assert op.dense_info(_lib.PADE_D1, B, 2, _lib.PADE_F32, _lib.PADE_F32)[2] == 48
+    assert op.dense_info(_lib.PADE_D1, B, 1, _lib.PADE_F64, _lib.PADE_F32)[2] == 48
+    assert op.dense_info(_lib.PADE_D1, F, 1, _lib.PADE_F32, _lib.PADE_F32)[2] == 8
+    assert op.dense_info(_lib.PADE_D1, B | _lib.PADE_DENSE_FULL, 1)[:2] == (512, 256)
+    # the second derivative's window starts two chunks back: no pattern
+    assert op.dense_info(_lib.PADE_D2, B, 1)[2] == 6 * op.dense_info(_lib.PADE_D2, B, 1)[0] // 16
+
+
 def test_dense_not_provided():
     f = torch.zeros(64 * 4 * 4, dtype=torch.float64, device=DEV)
     g = torch.zeros_like(f)

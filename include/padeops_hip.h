@@ -93,6 +93,23 @@ int pade_op_apply_axpby_f32(pade_op_t op, int oper, int axis, float alpha, const
 int pade_op_apply_host(pade_op_t op, int oper, int axis, const double *f, double *df, int na,
                        int nb, int bc1, int bcn);
 
+/* cf90 filter3 on complex data: replaces filter3_CMPLX (cf90.F90:1230-1251),
+ * the generic filter3's complex(rkind) binding.  f and df are complex128
+ * arrays (na, nb, n) in Fortran order, real and imaginary parts interleaved
+ * as Fortran stores complex(rkind); both parts are filtered along z.
+ * has_bc != 0 stands for the optional bc1_/bcn_ being present (the reference
+ * tests present(bcn_)); without them the codes are (0, 0).  As in the
+ * reference, a bounded filter forms the right-hand side of (bc1, bcn) and
+ * solves with the (0, 0) left-hand side (penta_nn).  A periodic cf90 applies
+ * its periodic filter (the reference reads penta_nn there as well, which a
+ * periodic cf90 never allocates).  Returns 21 for cd06/cd10 handles.
+ * Device arrays, asynchronous on `stream`; the _host form copies host arrays
+ * in and out and is synchronous. */
+int pade_op_filter3_complex(pade_op_t op, const void *f, void *df, int na, int nb, int bc1, int bcn, int has_bc,
+                            void *stream);
+int pade_op_filter3_complex_host(pade_op_t op, const void *f, void *df, int na, int nb, int bc1, int bcn,
+                                 int has_bc);
+
 /* Dense-operator form of a periodic operator: df = D f with D = A^{-1} B
  * precomputed in FP64 (circulant) and applied as an MFMA GEMM over all
  * pencils.  prec PADE_DENSE_F32 (FP32 operands and accumulation) or

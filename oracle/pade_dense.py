@@ -415,6 +415,30 @@ def dense_operator(kind, which, n, dx=1.0, periodic=True, bc1=0, bcn=0):
     return np.linalg.solve(A, B)
 
 
+def filter3_complex_operator(n, periodic=True, bc1=None, bcn=None):
+    """The (n x n) real operator that ``filter3`` applies to each part of a
+    complex(rkind) array, ``src/filters/cf90.F90:1230-1251``:
+
+    * the right-hand side uses (bc1, bcn) only when ``bcn_`` is present
+      (``:1242-1246``), else (0, 0);
+    * the solve always uses ``penta_nn`` (``:1249``), the (0, 0) left-hand
+      side, whatever codes formed the right-hand side.
+
+    For a periodic cf90 the reference solves with ``penta_nn`` too, which a
+    periodic operator never allocates (only ``LU``, ``:118-120``; the
+    ``penta_*`` arrays are allocated for bounded ones, ``:135``); the periodic
+    filter is used here instead (parity unpinned for that case: the
+    reference's result is undefined).  The bounded mixed operator is pinned in
+    tests/test_oracle_complex_filter.py: with (0, 0) it is the real filter,
+    and its rows away from the ends are the periodic filter's."""
+    if periodic:
+        return dense_operator("cf90", "filter", n, 1.0, True)
+    b1, bn = (0, 0) if bcn is None else (0 if bc1 is None else bc1, bcn)
+    A_nn, _ = assemble("cf90", "filter", n, 1.0, False, 0, 0)
+    _, B = assemble("cf90", "filter", n, 1.0, False, b1, bn)
+    return np.linalg.solve(A_nn, B)
+
+
 def apply_axis(D, f, axis):
     """Apply an (n x n) operator along a 0-based axis of a Fortran-ordered
     array ``f`` (shape (n1, n2, n3) in Fortran index order)."""

@@ -20,7 +20,7 @@ PADE_F64, PADE_F32 = 0, 1
 EXPORTS = [
     "pade_op_create", "pade_op_destroy", "pade_op_size", "pade_op_apply", "pade_op_apply_f32",
     "pade_op_apply_adjoint", "pade_op_apply_adjoint_f32", "pade_op_apply_axpby", "pade_op_apply_axpby_f32",
-    "pade_op_apply_host",
+    "pade_op_apply_host", "pade_op_filter3_complex", "pade_op_filter3_complex_host",
     "pade_op_dense_apply", "pade_op_dense_info", "pade_pencil_copy", "pade_permute3", "pade_plan_export",
     "pade_last_error", "pade_version",
 ]
@@ -58,6 +58,9 @@ def load():
     lib.pade_op_apply_axpby_f32.argtypes = [c_void_p, c_int, c_int, ctypes.c_float, c_void_p, ctypes.c_float,
                                             c_void_p, c_int, c_int, c_int, c_int, c_void_p]
     lib.pade_op_apply_host.argtypes = [c_void_p, c_int, c_int, dp, dp, c_int, c_int, c_int, c_int]
+    lib.pade_op_filter3_complex.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
+                                            c_void_p]
+    lib.pade_op_filter3_complex_host.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int]
     lib.pade_op_dense_apply.argtypes = [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_int,
                                         c_int, c_int, c_int, c_int, c_void_p]
     lib.pade_op_dense_info.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_int),

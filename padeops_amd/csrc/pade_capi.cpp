@@ -168,6 +168,12 @@ namespace {
 
 int plan_key_bc(const pade_op_s *op, int bc) { return op->kind == pade::CD06 ? 0 : bc; }
 
+// Plan key of the complex z filter's mixed operator (bounded cf90): the
+// right-hand side of boundary codes (bc1, bcn) solved with the (0, 0)
+// left-hand side, as cf90.F90:1248-1249 does (ComputeZRHS_CMPLX with the
+// caller's codes, then SolveZPenta_CMPLX with penta_nn).
+constexpr int kFilterRhsBc = 100 + pade::FILTER;
+
 // All (oper, bc1, bcn, adj) plans an operator can be asked for.
 int build_all_plans(pade_op_s *op)
 {
@@ -199,6 +205,25 @@ int build_all_plans(pade_op_s *op)
                 if (op->n >= 4 && build_variant(pl.mats, op->n, tab, pl.generic) == 0) pl.has_generic = true;
                 op->plans[{oper, b1, bn, adj}] = std::move(pl);
             }
+    }
+    if (op->kind == pade::CF90 && !op->periodic) {
+        pade::OpMatrices nn;
+        if (int rc = operator_matrices(op->kind, pade::FILTER, op->n, op->dx, false, 0, 0, false, nn))
+            return fail(rc, "operator construction failed");
+        for (auto [b1, bn] : bcs) {
+            if (b1 == 0 && bn == 0) continue;
+            Plan pl;
+            if (int rc = operator_matrices(op->kind, pade::FILTER, op->n, op->dx, false, b1, bn, false, pl.mats))
+                return fail(rc, "operator construction failed");
+            pl.mats.band = nn.band;      // (bc1, bcn) right-hand side, (0, 0) left-hand side
+            for (int M : {16, 32, 64}) {
+                if (op->n % M || op->n / M > pade::kMaxPartitions) continue;
+                Variant v;
+                if (build_variant(pl.mats, M, tab, v) == 0) pl.spike.push_back(v);
+            }
+            if (op->n >= 4 && build_variant(pl.mats, op->n, tab, pl.generic) == 0) pl.has_generic = true;
+            op->plans[{kFilterRhsBc, b1, bn, 0}] = std::move(pl);
+        }
     }
     op->tab_len = (long long)tab.size();
     hipError_t e = hipMalloc(&op->dtab64, tab.size() * sizeof(double));
@@ -409,6 +434,31 @@ int apply_impl(pade_op_t op, int oper, int axis, const T *f, T *df, int na, int 
     return apply_one<T>(op, it->second, axis, f, df, na, nb, adj, s, epi);
 }
 
+// cf90 filter3 on complex(rkind) data (cf90.F90:1230-1251).  A complex
+// (na, nb, n) array is a real (2 na, nb, n) one with the real and imaginary
+// parts interleaved along its first axis, and the z filter treats both
+// alike.  Bounded: the right-hand side of (bc1, bcn) when the caller passes
+// them (has_bc), else of (0, 0), always solved with the (0, 0) left-hand side
+// (penta_nn, cf90.F90:1249).  Periodic: the periodic filter (the reference
+// solves with penta_nn there too, which a periodic cf90 never allocates).
+int filter3_complex_impl(pade_op_t op, const double *f, double *df, int na, int nb, int bc1, int bcn, bool has_bc,
+                         hipStream_t s)
+{
+    if (!op) return fail(400, "null operator");
+    if (op->kind != pade::CF90) return fail(21, "filter3 on complex data is provided for cf90");
+    if (!f || !df) return fail(400, "null array");
+    if (na < 1 || nb < 1) return fail(400, "na and nb must be positive");
+    if (na > (1 << 29)) return fail(400, "na too large");
+    const int b1 = has_bc ? bc1 : 0, bn = has_bc ? bcn : 0;
+    auto ok = [](int b) { return b == 0 || b == 1 || b == -1; };
+    if (!ok(b1) || !ok(bn)) return fail(324, "Incorrect boundary specification (should be 0, 1 or -1)");
+    if (op->periodic || op->n == 1 || (b1 == 0 && bn == 0))
+        return apply_impl<double>(op, pade::FILTER, 3, f, df, 2 * na, nb, b1, bn, false, s);
+    auto it = op->plans.find({kFilterRhsBc, b1, bn, 0});
+    if (it == op->plans.end()) return fail(21, "no plan for this operation");
+    return apply_one<double>(op, it->second, 3, f, df, 2 * na, nb, false, s, Epilogue<double>());
+}
+
 }  // namespace
 
 extern "C" {
@@ -518,6 +568,38 @@ int pade_op_apply_host(pade_op_t op, int oper, int axis, const double *f, double
     e = hipMemcpy(f_d, f, bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy H2D");
     if (!rc) rc = apply_impl<double>(op, oper, axis, f_d, df_d, na, nb, bc1, bcn, false, nullptr);
+    if (!rc) {
+        e = hipMemcpy(df, df_d, bytes, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy D2H");
+    }
+    hipFree(f_d);
+    hipFree(df_d);
+    return rc;
+}
+
+int pade_op_filter3_complex(pade_op_t op, const void *f, void *df, int na, int nb, int bc1, int bcn, int has_bc,
+                            void *stream)
+{
+    return filter3_complex_impl(op, static_cast<const double *>(f), static_cast<double *>(df), na, nb, bc1, bcn,
+                                has_bc != 0, (hipStream_t)stream);
+}
+
+int pade_op_filter3_complex_host(pade_op_t op, const void *f, void *df, int na, int nb, int bc1, int bcn,
+                                 int has_bc)
+{
+    if (!op) return fail(400, "null operator");
+    if (!f || !df) return fail(400, "null array");
+    if (na < 1 || nb < 1) return fail(400, "na and nb must be positive");
+    const size_t bytes = 2 * sizeof(double) * (size_t)na * nb * op->n;
+    double *df_d = nullptr, *f_d = nullptr;
+    hipError_t e = hipMalloc(&f_d, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    e = hipMalloc(&df_d, bytes);
+    if (e != hipSuccess) { hipFree(f_d); return hip_fail(e, "hipMalloc"); }
+    int rc = 0;
+    e = hipMemcpy(f_d, f, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy H2D");
+    if (!rc) rc = filter3_complex_impl(op, f_d, df_d, na, nb, bc1, bcn, has_bc != 0, nullptr);
     if (!rc) {
         e = hipMemcpy(df, df_d, bytes, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy D2H");

@@ -1,8 +1,8 @@
 ! Drop-in for the reference's cf90 module (src/filters/cf90.F90): same module,
 ! type and procedure names, argument orders and optional arguments; the
 ! arithmetic runs in libpadeops_hip.so.  filter3 is generic as in the
-! reference; its real-array form is provided (the complex form used on
-! spectral data, cf90.F90:1230-1251, is outside this library).
+! reference, with its real (cf90.F90:1160-1228) and complex (spectral data,
+! cf90.F90:1230-1251) forms.
 module cf90stuff
     use iso_c_binding
     use padeops_c
@@ -23,7 +23,8 @@ module cf90stuff
         procedure :: filter1
         procedure :: filter2
         procedure, private :: filter3_REAL
-        generic :: filter3 => filter3_REAL
+        procedure, private :: filter3_CMPLX
+        generic :: filter3 => filter3_REAL, filter3_CMPLX
         procedure :: apply_dev
     end type
 
@@ -88,6 +89,27 @@ contains
         integer, optional, intent(in) :: bc1_, bcn_
         call pade_apply_host(this%h, PADE_FILTER, 3_c_int, f, df, na, nb, pade_bc(bc1_, "bc1"), &
                              pade_bc(bcn_, "bcn"), "cf90%filter3")
+    end subroutine
+
+    ! cf90.F90:1230-1251: both parts filtered along z; the right-hand side of
+    ! (bc1_, bcn_) when bcn_ is present, else (0, 0), solved with the (0, 0)
+    ! left-hand side as the reference does
+    subroutine filter3_CMPLX(this, f, df, na, nb, bc1_, bcn_)
+        class(cf90), intent(in) :: this
+        integer, intent(in) :: na, nb
+        complex(rkind), dimension(na, nb, this%n), intent(in) :: f
+        complex(rkind), dimension(na, nb, this%n), intent(out) :: df
+        integer, optional, intent(in) :: bc1_, bcn_
+        integer(c_int) :: b1, bn, has
+        if (.not. c_associated(this%h)) call pade_fail("cf90%filter3: operator used before init", 400)
+        b1 = 0; bn = 0; has = 0
+        if (present(bcn_)) then
+            has = 1
+            b1 = int(pade_bc(bc1_, "bc1"), c_int)
+            bn = int(pade_bc(bcn_, "bcn"), c_int)
+        end if
+        call pade_check(pade_op_filter3_complex_host(this%h, f, df, int(na, c_int), int(nb, c_int), b1, bn, has), &
+                        "cf90%filter3")
     end subroutine
 
     ! Device-resident arrays (no reference counterpart).

@@ -13,7 +13,7 @@ module padeops_c
     integer, parameter, public :: pade_clen = 256
 
     public :: pade_op_create, pade_op_destroy, pade_op_size, pade_op_apply, pade_op_apply_adjoint
-    public :: pade_op_apply_host, pade_last_error, pade_version
+    public :: pade_op_apply_host, pade_op_filter3_complex_host, pade_last_error, pade_version
     public :: pade_check, pade_fail, pade_apply_host, pade_apply_device, pade_bc
 
     interface
@@ -56,6 +56,15 @@ module padeops_c
             integer(c_int), value :: oper, axis, na, nb, bc1, bcn
             real(c_double), intent(in) :: f(*)
             real(c_double), intent(out) :: df(*)
+        end function
+
+        integer(c_int) function pade_op_filter3_complex_host(op, f, df, na, nb, bc1, bcn, has_bc) &
+                bind(C, name="pade_op_filter3_complex_host")
+            import :: c_int, c_ptr, c_double_complex
+            type(c_ptr), value :: op
+            integer(c_int), value :: na, nb, bc1, bcn, has_bc
+            complex(c_double_complex), intent(in) :: f(*)
+            complex(c_double_complex), intent(out) :: df(*)
         end function
 
         type(c_ptr) function pade_last_error() bind(C, name="pade_last_error")

@@ -42,6 +42,12 @@ def as_fortran(t):
     return out
 
 
+def _is_complex(a):
+    if isinstance(a, np.ndarray):
+        return np.iscomplexobj(a)
+    return torch is not None and isinstance(a, torch.Tensor) and a.is_complex()
+
+
 def _is_dense_block(t):
     if t.is_contiguous():
         return True
@@ -266,4 +272,46 @@ class cf90(_CompactOp):
         return self._apply(PADE_FILTER, 2, f, df, na, nb, bc1, bcn)
 
     def filter3(self, f, df, na, nb, bc1=None, bcn=None):
+        """Real data: filter along z.  Complex data (torch.complex128 on the
+        GPU or numpy complex128 on the host): the reference's filter3_CMPLX
+        (cf90.F90:1230-1251) -- both parts filtered; a bounded filter takes
+        the right-hand side of (bc1, bcn) when bcn is given, else of (0, 0),
+        and always solves with the (0, 0) left-hand side, as the reference
+        does."""
+        if _is_complex(f) or _is_complex(df):
+            return self._filter3_complex(f, df, na, nb, bc1, bcn)
         return self._apply(PADE_FILTER, 3, f, df, na, nb, bc1, bcn)
+
+    def _filter3_complex(self, f, df, na, nb, bc1, bcn):
+        if not self._h:
+            raise PadeError(400, "operator used before init")
+        lib = _lib.load()
+        count = self._n * int(na) * int(nb)
+        has_bc = bcn is not None
+        b1 = int(bc1) if (has_bc and bc1 is not None) else 0
+        bn = int(bcn) if has_bc else 0
+        if isinstance(f, np.ndarray):
+            if f.dtype != np.complex128 or df.dtype != np.complex128:
+                raise TypeError("complex host arrays must be complex128")
+            if f.size != count or df.size != count:
+                raise ValueError(f"array size does not match n*na*nb = {count}")
+            fa = np.asfortranarray(f) if f.ndim > 1 else np.ascontiguousarray(f)
+            if not (df.flags.f_contiguous or df.ndim == 1):
+                raise ValueError("df must be a Fortran-ordered array")
+            _lib.check(lib.pade_op_filter3_complex_host(self._h, ctypes.c_void_p(fa.ctypes.data),
+                                                        ctypes.c_void_p(df.ctypes.data), int(na), int(nb),
+                                                        b1, bn, int(has_bc)))
+            return df
+        if f.dtype != torch.complex128 or df.dtype != torch.complex128:
+            raise TypeError("complex device arrays must be complex128")
+        if not (f.is_cuda and df.is_cuda):
+            raise TypeError("device arrays must live on the GPU")
+        if f.numel() != count or df.numel() != count:
+            raise ValueError(f"array size {f.numel()} does not match n*na*nb = {count}")
+        if not (_is_dense_block(f) and _is_dense_block(df)):
+            raise ValueError("arrays must be dense blocks in Fortran order")
+        if f.data_ptr() == df.data_ptr():
+            raise ValueError("f and df must not alias")
+        _lib.check(lib.pade_op_filter3_complex(self._h, ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(df.data_ptr()),
+                                               int(na), int(nb), b1, bn, int(has_bc), _stream_ptr()))
+        return df

@@ -14,6 +14,7 @@ program test_shim
     integer, parameter :: n1 = 24, n2 = 20, n3 = 16
     real(rk), parameter :: pi = 4.0_rk * atan(1.0_rk)
     real(rk), allocatable :: f(:, :, :), g(:, :, :), p1(:)
+    complex(rk), allocatable :: fc(:, :, :), gc(:, :, :)
     character(len=512) :: path
     integer :: u, i, j, k, ierr
     type(cd10) :: c10
@@ -81,6 +82,16 @@ program test_shim
     ierr = c90%init(n3, .true.)
     call c90%filter3(f, g, n1, n2)
     call put("cf90_p_f3", g)
+    ! complex filter3 (spectral data): (1, 1) right-hand side, (0, 0) solve
+    allocate(fc(n1, n2, n3), gc(n1, n2, n3))
+    fc = cmplx(f, f(n1:1:-1, :, :), kind=rk)
+    ierr = c90%init(n3, .false.)
+    call c90%filter3(fc, gc, n1, n2, 1, 1)
+    call put("cf90_cb3_11_re", real(gc, kind=rk))
+    call put("cf90_cb3_11_im", aimag(gc))
+    call c90%filter3(fc, gc, n1, n2)
+    call put("cf90_cb3_re", real(gc, kind=rk))
+    call put("cf90_cb3_im", aimag(gc))
     call c90%destroy()
 
     call der%init(n1, n2, n3, 2 * pi / n1, 1.0_rk / (n2 - 1), 1.0_rk / (n3 - 1), .true., .false., .false., &

@@ -82,6 +82,20 @@ def test_fortran_shim_outputs(records, tag, kind, which, axis, periodic, bc):
     assert err < 1e-12, (tag, err)
 
 
+@pytest.mark.parametrize("tag,bc", [("cf90_cb3_11", (1, 1)), ("cf90_cb3", (None, None))])
+def test_fortran_complex_filter3(records, tag, bc):
+    """filter3_CMPLX (cf90.F90:1230-1251) through the generic filter3 on
+    complex(rkind) data: each part through the (bc1, bcn) right-hand side and
+    the (0, 0) solve."""
+    f = records["input"]
+    re, im = f, f[::-1, :, :]
+    D = pd.filter3_complex_operator(f.shape[2], False, *bc)
+    for part, src in (("_re", re), ("_im", im)):
+        expect = pd.apply_axis(D, src, 2)
+        got = records[tag + part]
+        assert np.abs(got - expect).max() / np.abs(expect).max() < 1e-12, (tag, part)
+
+
 def test_fortran_cd06der2_pencil(records):
     f = records["input"][:, 2, 4]
     n = f.shape[0]

@@ -109,6 +109,48 @@ def test_throughput_variants(kind, which, periodic, bc, n, axis):
     assert relerr(out, ref) < TOL64, (kind, which, periodic, bc, n, axis, relerr(out, ref))
 
 
+@pytest.mark.parametrize("periodic", [True, False])
+@pytest.mark.parametrize("bc", [None, (0, 0), (1, 1), (-1, 1), (1, 0)])
+@pytest.mark.parametrize("n", [64, 200])
+def test_filter3_complex(periodic, bc, n):
+    """filter3 on complex128 data (cf90.F90:1230-1251) through the C ABI,
+    device and host arrays, against the oracle's operator on each part."""
+    na, nb = 10, 7
+    rng = np.random.default_rng(n + (3 if periodic else 5))
+    f = np.asfortranarray(rng.standard_normal((na, nb, n)) + 1j * rng.standard_normal((na, nb, n)))
+    op = po.cf90()
+    assert op.init(n, periodic) == 0
+    b1, bn = (None, None) if bc is None else bc
+    D = pd.filter3_complex_operator(n, periodic, b1, bn)
+    ref = pd.apply_axis(D, f.real, 2) + 1j * pd.apply_axis(D, f.imag, 2)
+    ft = po.as_fortran(torch.tensor(f, device=DEV))
+    out = po.fortran_empty(f.shape, dtype=torch.complex128, device=DEV)
+    op.filter3(ft, out, na, nb, b1, bn)
+    torch.cuda.synchronize()
+    assert relerr(out.cpu().numpy(), ref) < TOL64
+    hout = np.zeros_like(f, order="F")
+    op.filter3(f, hout, na, nb, b1, bn)
+    assert relerr(hout, ref) < TOL64
+
+
+def test_filter3_complex_errors():
+    op = po.cd10()
+    assert op.init(64, 0.1, True) == 0
+    f = torch.zeros((4, 4, 64), dtype=torch.complex128, device=DEV)
+    lib = _lib.load()
+    import ctypes
+    rc = lib.pade_op_filter3_complex(op.handle, ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(f.data_ptr()),
+                                     4, 4, 0, 0, 0, None)
+    assert rc == 21
+    fl = po.cf90()
+    assert fl.init(64, False) == 0
+    g = po.fortran_empty((4, 4, 64), dtype=torch.complex128, device=DEV)
+    fa = po.fortran_empty((4, 4, 64), dtype=torch.complex128, device=DEV)
+    with pytest.raises(po.PadeError) as e:
+        fl.filter3(fa, g, 4, 4, 2, 0)
+    assert e.value.code == 324
+
+
 @pytest.mark.parametrize("init_bc", [(0, 0), (1, 1), (0, 1)])
 @pytest.mark.parametrize("axis", [1, 2, 3])
 def test_cd06_bounded(init_bc, axis):

@@ -34,8 +34,24 @@
 extern "C" {
 #endif
 
-enum pade_kind { PADE_CD06 = 1, PADE_CD10 = 2, PADE_CF90 = 3 };
-enum pade_oper { PADE_D1 = 1, PADE_D2 = 2, PADE_FILTER = 3 };
+enum pade_kind { PADE_CD06 = 1, PADE_CD10 = 2, PADE_CF90 = 3, PADE_CD06STAGG = 4 };
+/* PADE_CD06STAGG (periodic only; cd06stagg.F90:170-195 init_periodic, n > 4)
+ * takes the staggered operations, along z (axis 3) only, on edge (E) arrays of
+ * n + 1 planes (the last a copy of the first) and cell (C) arrays of n:
+ *   PADE_D1_E2C      ddz_E2C      (cd06stagg.F90:820)
+ *   PADE_D1_C2E      ddz_C2E      (:850)
+ *   PADE_D2_C2C      d2dz2_C2C    (:995)
+ *   PADE_D2_E2E      d2dz2_E2E    (:1027)
+ *   PADE_INTERP_E2C  InterpZ_E2C  (:928)
+ *   PADE_INTERP_C2E  InterpZ_C2E  (:960)
+ * through pade_op_apply / _f32 / _axpby (f and df sized by their grid);
+ * E outputs get their last plane copied from the first, as the reference
+ * does.  No adjoint and no dense form. */
+enum pade_oper {
+    PADE_D1 = 1, PADE_D2 = 2, PADE_FILTER = 3,
+    PADE_D1_E2C = 11, PADE_D1_C2E = 12, PADE_D2_C2C = 13, PADE_D2_E2E = 14,
+    PADE_INTERP_E2C = 15, PADE_INTERP_C2E = 16
+};
 enum pade_dense_prec { PADE_DENSE_F32 = 1, PADE_DENSE_BF16X3 = 2, PADE_DENSE_FULL = 16 };
 enum pade_dtype { PADE_F64 = 0, PADE_F32 = 1 };
 

@@ -22,7 +22,12 @@ Pinning (tests/test_oracle_*.py):
   * cd06 second derivative -> the reference has no implementation
     (``src/derivatives/cd06.F90:330-343,492-513,745-773,841-853``); the
     operator defined here is pinned only by its closed form ("parity unpinned"
-    with respect to reference output).
+    with respect to reference output),
+  * periodic staggered operators (``src/derivatives/cd06stagg.F90``) -> sixth
+    order of accuracy on smooth periodic data at edges and cells,
+  * complex z filter (``src/filters/cf90.F90:1230-1251``) -> composition of
+    the pinned A (0, 0) and B (bc1, bcn); its periodic case is unpinned (the
+    reference reads an unallocated factorization there).
 """
 from __future__ import annotations
 
@@ -487,3 +492,57 @@ def spectral_apply(kind, which, f, axis, dx):
     F = np.fft.fft(f, axis=axis)
     out = np.fft.ifft(F * lam.reshape(shape), axis=axis)
     return out.real
+
+
+# ----------------------------------------------------------------------------
+# Periodic staggered operators (src/derivatives/cd06stagg.F90)
+# ----------------------------------------------------------------------------
+
+# RHS taps as {input offset: weight}, output point i of a cell array lying
+# between input edges i and i+1, output edge j between input cells j-1 and j;
+# the LHS is the cyclic tridiagonal (alpha, 1, alpha).
+def _stagg_scheme(op):
+    if op in ("d1_e2c", "d1_c2e"):
+        a, b = 63.0 / 62.0, (17.0 / 62.0) / 3.0              # cd06stagg.F90:307,355
+        taps = ({0: -a, 1: a, -1: -b, 2: b} if op == "d1_e2c"      # :313-320
+                else {-1: -a, 0: a, -2: -b, 1: b})                   # :361-373
+        return 9.0 / 62.0, taps, 1                             # alpha :174
+    if op in ("d2_c2c", "d2_e2e"):
+        a, b = 12.0 / 11.0, (3.0 / 11.0) / 4.0               # :411,471
+        return 2.0 / 11.0, {0: -2.0 * (a + b), 1: a, -1: a, 2: b, -2: b}, 2   # :417-429, :175
+    if op in ("interp_e2c", "interp_c2e"):
+        a, b = 1.5 / 2.0, 0.1 / 2.0                           # :532-536,578-582
+        taps = ({0: a, 1: a, -1: b, 2: b} if op == "interp_e2c"    # :538-545
+                else {-1: a, 0: a, -2: b, 1: b})                     # :585-597
+        return 3.0 / 10.0, taps, 0                             # alpha :176
+    raise ValueError(op)
+
+
+def stagg_operator(op, n, dx):
+    """(n x n) D = A^{-1} B of a periodic staggered operation on the n distinct
+    points of a period (an edge array's (n+1)-th plane repeats its first)."""
+    al, taps, p = _stagg_scheme(op)
+    A = np.zeros((n, n))
+    B = np.zeros((n, n))
+    for i in range(n):
+        A[i, i] = 1.0
+        A[i, (i - 1) % n] += al
+        A[i, (i + 1) % n] += al
+        for o, v in taps.items():
+            B[i, (i + o) % n] += v
+    return np.linalg.solve(A, B * dx ** (-p))
+
+
+STAGG_EDGE_IN = {"d1_e2c": True, "d1_c2e": False, "d2_c2c": False, "d2_e2e": True,
+                 "interp_e2c": True, "interp_c2e": False}
+STAGG_EDGE_OUT = {"d1_e2c": False, "d1_c2e": True, "d2_c2c": False, "d2_e2e": True,
+                  "interp_e2c": False, "interp_c2e": True}
+
+
+def stagg_apply(op, f, dx):
+    """Apply along z (last axis) of a Fortran-ordered (n1, n2, n or n+1) array."""
+    n = f.shape[2] - (1 if STAGG_EDGE_IN[op] else 0)
+    out = apply_axis(stagg_operator(op, n, dx), f[:, :, :n], 2)
+    if STAGG_EDGE_OUT[op]:
+        out = np.concatenate([out, out[:, :, :1]], axis=2)
+    return np.asfortranarray(out)

@@ -145,7 +145,15 @@ int operator_matrices(int kind, int oper, int n, double dx, bool periodic, int b
 bool oper_valid(int kind, int oper)
 {
     if (kind == pade::CF90) return oper == pade::FILTER;
+    if (kind == pade::CD06STAGG) return oper >= pade::D1_E2C && oper <= pade::INTERP_C2E;
     return oper == pade::D1 || oper == pade::D2;
+}
+
+// Staggered operations whose output lives on the n + 1 edge planes (the
+// last a copy of the first, cd06stagg.F90:859,969,1036).
+bool edge_output(int kind, int oper)
+{
+    return kind == pade::CD06STAGG && (oper == pade::D1_C2E || oper == pade::D2_E2E || oper == pade::INTERP_C2E);
 }
 
 }  // namespace
@@ -180,6 +188,8 @@ int build_all_plans(pade_op_s *op)
     std::vector<double> tab;
     std::vector<int> opers;
     if (op->kind == pade::CF90) opers = {pade::FILTER};
+    else if (op->kind == pade::CD06STAGG)
+        opers = {pade::D1_E2C, pade::D1_C2E, pade::D2_C2C, pade::D2_E2E, pade::INTERP_E2C, pade::INTERP_C2E};
     else opers = {pade::D1, pade::D2};
     std::vector<std::pair<int, int>> bcs;
     if (op->periodic || op->kind == pade::CD06) bcs = {{0, 0}};
@@ -191,6 +201,7 @@ int build_all_plans(pade_op_s *op)
         for (auto [b1, bn] : bcs)
             for (int adj = 0; adj < 2; adj++) {
                 if (adj && op->n < 16) continue;  // adjoint needs n >= 16 (boundary rows of B^T)
+                if (adj && op->kind == pade::CD06STAGG) continue;   // no staggered adjoint
                 const int cb1 = op->kind == pade::CD06 ? op->init_bc1 : b1;
                 const int cbn = op->kind == pade::CD06 ? op->init_bcn : bn;
                 Plan pl;
@@ -398,6 +409,10 @@ int apply_impl(pade_op_t op, int oper, int axis, const T *f, T *df, int na, int 
     if (axis < 1 || axis > 3) return fail(400, "axis must be 1, 2 or 3");
     if (na < 1 || nb < 1) return fail(400, "na and nb must be positive");
     if (!oper_valid(op->kind, oper)) return fail(21, "operation not provided for this operator kind");
+    if (op->kind == pade::CD06STAGG) {
+        if (axis != 3) return fail(400, "the staggered operators act along z (axis 3)");
+        if (adj) return fail(21, "no adjoint of the staggered operators");
+    }
     if (op->kind != pade::CD06) {
         auto ok = [](int b) { return b == 0 || b == 1 || b == -1; };
         if (!ok(bc1) || !ok(bcn))
@@ -431,7 +446,14 @@ int apply_impl(pade_op_t op, int oper, int axis, const T *f, T *df, int na, int 
     auto it = op->plans.find({oper, k1, kn, adj ? 1 : 0});
     if (it == op->plans.end())
         return fail(21, adj ? "adjoint needs n >= 16" : "no plan for this operation");
-    return apply_one<T>(op, it->second, axis, f, df, na, nb, adj, s, epi);
+    const int rc = apply_one<T>(op, it->second, axis, f, df, na, nb, adj, s, epi);
+    if (!rc && edge_output(op->kind, oper)) {
+        // an edge array's last plane repeats its first (cd06stagg.F90:859,969,1036)
+        const size_t plane = (size_t)na * nb;
+        hipError_t e = hipMemcpyAsync(df + plane * op->n, df, sizeof(T) * plane, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "edge plane copy");
+    }
+    return rc;
 }
 
 // cf90 filter3 on complex(rkind) data (cf90.F90:1230-1251).  A complex
@@ -471,11 +493,13 @@ int pade_op_create(int kind, int n, double dx, int periodic, int bc1, int bcn, p
 {
     if (!out) return fail(400, "null output handle");
     *out = nullptr;
-    if (kind < pade::CD06 || kind > pade::CF90) return fail(400, "unknown operator kind");
+    if (kind < pade::CD06 || kind > pade::CD06STAGG) return fail(400, "unknown operator kind");
     if (n < 1) return fail(400, "n must be positive");
     if (kind != pade::CF90 && !(dx > 0.0) && n > 1) return fail(400, "dx must be positive");
     if (int ierr = pade::init_error(kind, n, periodic != 0))
         return fail(ierr, "grid too small for this operator");
+    if (kind == pade::CD06STAGG && !periodic)
+        return fail(21, "the non-periodic staggered operators are not provided");
     auto op = std::make_unique<pade_op_s>();
     op->kind = kind;
     op->n = n;
@@ -558,18 +582,23 @@ int pade_op_apply_host(pade_op_t op, int oper, int axis, const double *f, double
     if (!op) return fail(400, "null operator");
     if (!f || !df) return fail(400, "null array");
     if (na < 1 || nb < 1) return fail(400, "na and nb must be positive");
-    const size_t bytes = sizeof(double) * (size_t)na * nb * op->n;
+    // staggered edge arrays hold one plane more than the pencil length
+    const bool stagg = op->kind == pade::CD06STAGG;
+    const bool e_in = stagg && (oper == pade::D1_E2C || oper == pade::D2_E2E || oper == pade::INTERP_E2C);
+    const bool e_out = edge_output(op->kind, oper);
+    const size_t plane = sizeof(double) * (size_t)na * nb;
+    const size_t in_bytes = plane * (op->n + (e_in ? 1 : 0)), out_bytes = plane * (op->n + (e_out ? 1 : 0));
     double *df_d = nullptr, *f_d = nullptr;
-    hipError_t e = hipMalloc(&f_d, bytes);
+    hipError_t e = hipMalloc(&f_d, in_bytes);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc");
-    e = hipMalloc(&df_d, bytes);
+    e = hipMalloc(&df_d, out_bytes);
     if (e != hipSuccess) { hipFree(f_d); return hip_fail(e, "hipMalloc"); }
     int rc = 0;
-    e = hipMemcpy(f_d, f, bytes, hipMemcpyHostToDevice);
+    e = hipMemcpy(f_d, f, in_bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy H2D");
     if (!rc) rc = apply_impl<double>(op, oper, axis, f_d, df_d, na, nb, bc1, bcn, false, nullptr);
     if (!rc) {
-        e = hipMemcpy(df, df_d, bytes, hipMemcpyDeviceToHost);
+        e = hipMemcpy(df, df_d, out_bytes, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy D2H");
     }
     hipFree(f_d);
@@ -622,6 +651,7 @@ int pade_op_dense_apply(pade_op_t op, int oper, int axis, const void *f, int f_d
         if (!ok(bc1) || !ok(bcn))
             return fail(324, "Incorrect boundary specification (should be 0, 1 or -1)");
     }
+    if (op->kind == pade::CD06STAGG) return fail(21, "no dense form of the staggered operators");
     const bool full = (prec & PADE_DENSE_FULL) != 0;
     const int pbase = prec & ~PADE_DENSE_FULL;
     if (pbase != PADE_DENSE_F32 && pbase != PADE_DENSE_BF16X3) return fail(400, "unknown dense precision");
@@ -646,6 +676,7 @@ int pade_op_dense_info(pade_op_t op, int oper, int prec, int axis, int f_dtype, 
                        int *halfwidth, int *products)
 {
     if (!op || !window || !halfwidth || !products) return fail(400, "null argument");
+    if (op->kind == pade::CD06STAGG) return fail(21, "no dense form of the staggered operators");
     if (axis < 1 || axis > 3) return fail(400, "axis must be 1, 2 or 3");
     if ((f_dtype != PADE_F64 && f_dtype != PADE_F32) || (df_dtype != PADE_F64 && df_dtype != PADE_F32))
         return fail(400, "dtype must be PADE_F64 or PADE_F32");
@@ -687,7 +718,7 @@ int pade_plan_export(int kind, int oper, int n, double dx, int periodic, int bc1
                      int M, double *tab, long long tab_len, int *meta)
 {
     if (!meta) return fail(400, "null meta");
-    if (kind < pade::CD06 || kind > pade::CF90) return fail(400, "unknown operator kind");
+    if (kind < pade::CD06 || kind > pade::CD06STAGG) return fail(400, "unknown operator kind");
     if (!oper_valid(kind, oper)) return fail(21, "operation not provided for this operator kind");
     if (int ierr = pade::init_error(kind, n, periodic != 0)) return fail(ierr, "grid too small");
     if (n < 4) return fail(21, "no table for n < 4");

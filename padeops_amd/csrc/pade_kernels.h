@@ -43,7 +43,7 @@ struct ApplyArgs {
     T alpha, beta;
 };
 
-size_t spike_lds_bytes(size_t tsize, int M, int P, int PW, bool adj, int mode, int coef_len);
+size_t spike_lds_bytes(size_t tsize, int M, int P, int PW, bool adj, int mode, int coef_len, bool bnd);
 
 template <typename T>
 hipError_t launch_spike(const ApplyArgs<T> &a, int M, int PW, bool adj, int mode, hipStream_t s);

@@ -134,21 +134,22 @@ constexpr int kHaloChunk = 8;
 __host__ __device__ constexpr int xtile_row(int tsize, int M) { return xtile_points(tsize, M) + 16 / tsize; }
 
 // LDS layout (elements of T):
-//   tips  [4P][NPEN]
+//   tips  [4P][NPEN]               (whole-pencil x waves: in their tile)
 //   halo  [8P][NPEN]               adjoint, y/z pencils (partition end values)
-//   bnd   [256]                    boundary rows (left 128, right 128)
+//   bnd   [256]                    bounded only: boundary rows (left 128, right 128)
 //   coef  [coef_len]               coefficient classes + reduced-system rows
 //   xt    [waves][64][XT + VL]     x pencils only; the x adjoint's halo
 //                                  exchange uses the start of this area (the
 //                                  tiles are idle between the load and store
 //                                  transposes), which keeps the M = 64 FP64
 //                                  adjoint at two workgroups per CU
-__host__ __device__ inline int lds_elems(int M, int P, int PW, bool adj, int mode, int coef_len, int tsize)
+__host__ __device__ inline int lds_elems(int M, int P, int PW, bool adj, int mode, int coef_len, int tsize, bool bnd)
 {
     const int npen = 64 / PW;
     const int waves = (P + PW - 1) / PW;
+    const bool whole = mode == 1 && PW >= 4;      // tips in the tile area
     // (x pencils exchange the adjoint halo through their transpose tiles)
-    int e = (adj && mode != 1 ? 12 : 4) * P * npen + 256 + coef_len;
+    int e = (whole ? 0 : (adj && mode != 1 ? 12 : 4) * P * npen) + (bnd ? 256 : 0) + coef_len;
     if (mode == 1) {
         const int vl = 16 / tsize;
         e = (e + vl - 1) / vl * vl;           // 16-byte aligned tile
@@ -157,9 +158,9 @@ __host__ __device__ inline int lds_elems(int M, int P, int PW, bool adj, int mod
     return e;
 }
 
-size_t spike_lds_bytes(size_t tsize, int M, int P, int PW, bool adj, int mode, int coef_len)
+size_t spike_lds_bytes(size_t tsize, int M, int P, int PW, bool adj, int mode, int coef_len, bool bnd)
 {
-    return (size_t)lds_elems(M, P, PW, adj, mode, coef_len, (int)tsize) * tsize;
+    return (size_t)lds_elems(M, P, PW, adj, mode, coef_len, (int)tsize, bnd) * tsize;
 }
 
 template <typename T, int M>
@@ -250,9 +251,9 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
     const bool edge_left = BND && p == 0;
     const bool edge_right = BND && p == P - 1;
 
-    T *tips = smem;                                     // [4P][NPEN]
-    T *bnd = smem + (ADJ && MODE != 1 ? 12 : 4) * P * NPEN;   // [256]
-    T *coef = bnd + 256;                                // [coef_len]
+    constexpr bool WHOLE = MODE == 1 && PW >= 4;
+    T *bnd = smem + (WHOLE ? 0 : (ADJ && MODE != 1 ? 12 : 4) * P * NPEN);   // [256] (bounded)
+    T *coef = bnd + (BND ? 256 : 0);                    // [coef_len]
     constexpr int XT = xtile_points(sizeof(T), M), PAD = xtile_row(sizeof(T), M);
     constexpr int VL = 16 / sizeof(T);                  // elements per 16-byte lane
     constexpr int LPR = XT / VL;                        // lanes per tile row
@@ -261,6 +262,9 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
     T *xt = smem + xt0 + wave * 64 * PAD;
     // [8P][NPEN] adjoint halo: 8P * NPEN <= waves * 64 * PAD since PAD >= 8
     T *halo = MODE == 1 ? smem + xt0 : smem + 4 * P * NPEN;
+    // [4P][NPEN] tips; a whole-pencil wave (its own workgroup) keeps them in
+    // its tile, idle between the load and store transposes
+    T *tips = WHOLE ? smem + xt0 : smem;
 
     // ----------------------------------------------- 0. coefficients -> LDS
     // (broadcast LDS reads measured faster than scalar loads of the same
@@ -714,7 +718,7 @@ static hipError_t launch_spike_t(const ApplyArgs<T> &a, hipStream_t s)
     constexpr int NPEN = 64 / PW;
     const long long nblk = (a.npencils + NPEN - 1) / NPEN;
     const int waves = (a.P + PW - 1) / PW;
-    const size_t lds = spike_lds_bytes(sizeof(T), M, a.P, PW, ADJ, MODE, a.coef_len);
+    const size_t lds = spike_lds_bytes(sizeof(T), M, a.P, PW, ADJ, MODE, a.coef_len, !a.periodic);
     // the accumulate epilogue exists for the forward operator only
     if (ADJ && a.axpby) return hipErrorInvalidValue;
     const bool epi = !ADJ && a.axpby;

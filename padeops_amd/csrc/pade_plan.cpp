@@ -75,6 +75,35 @@ Interior interior(int kind, int oper)
         s.b[1] = s.b[7] = 4.0e-5;
         s.b[0] = s.b[8] = -5.0e-6;
         s.H = 4; s.parity = 1; s.dxpow = 0;
+    } else if (kind == CD06STAGG) {
+        // periodic staggered sixth-order scheme, cd06stagg.F90:170-195 (LHS
+        // alphas) and the ComputeZ*_periodic RHS routines: output point i of a
+        // cell array sits between input edges i and i + 1, output edge j between
+        // input cells j - 1 and j.  Index offsets below are input minus output.
+        double al = 0.0;
+        if (oper == D1_E2C || oper == D1_C2E) {
+            al = 9.0 / 62.0;                                    // :174
+            const double a = 63.0 / 62.0, b = (17.0 / 62.0) / 3.0;   // :307, :355
+            const int o = oper == D1_E2C ? 0 : -1;              // E2C :313-320, C2E :361-373
+            s.b[4 + o] = -a; s.b[5 + o] = a; s.b[3 + o] = -b; s.b[6 + o] = b;
+            s.parity = -1; s.dxpow = 1;
+        } else if (oper == D2_C2C || oper == D2_E2E) {
+            al = 2.0 / 11.0;                                    // :175
+            const double a = 12.0 / 11.0, b = (3.0 / 11.0) / 4.0;  // :411, :471
+            s.b[4] = -2.0 * (a + b); s.b[3] = s.b[5] = a; s.b[2] = s.b[6] = b;
+            s.parity = 1; s.dxpow = 2;
+        } else if (oper == INTERP_E2C || oper == INTERP_C2E) {
+            al = 3.0 / 10.0;                                    // :176
+            const double a = 1.5 / 2.0, b = 0.1 / 2.0;          // :532-536, :578-582
+            const int o = oper == INTERP_E2C ? 0 : -1;          // E2C :538-545, C2E :585-597
+            s.b[4 + o] = s.b[5 + o] = a; s.b[3 + o] = s.b[6 + o] = b;
+            s.parity = 1; s.dxpow = 0;
+        } else {
+            s.H = -1;
+            return s;
+        }
+        s.a[1] = al; s.a[2] = 1.0; s.a[3] = al;
+        s.H = 2;
     } else {
         s.H = -1;
     }
@@ -200,6 +229,8 @@ int init_error(int kind, int n, bool periodic)
     if (kind == CD06) return periodic ? (n >= 6 ? 0 : 3) : (n >= 8 ? 0 : 2);
     if (kind == CD10) return n >= 8 ? 0 : 2;
     if (kind == CF90) return n >= 10 ? 0 : 7;
+    // cd06stagg.F90:182-184: GracefulExit 21 for nx <= 4
+    if (kind == CD06STAGG) return n > 4 ? 0 : 21;
     return 1;
 }
 
@@ -209,6 +240,7 @@ int build_operator(int kind, int oper, int n, double dx, bool periodic, int bc1,
     Interior s = interior(kind, oper);
     if (s.H < 0) return 21;
     if (kind == CD06 && oper == D2 && !periodic) return 21;  // composed from d1 by the caller
+    if (kind == CD06STAGG && !periodic) return 21;           // periodic staggered operators only
     if (kind != CD06 && !periodic) {
         auto ok = [](int b) { return b == 0 || b == 1 || b == -1; };
         if (!ok(bc1) || !ok(bcn)) return 324;

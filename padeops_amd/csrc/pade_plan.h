@@ -17,8 +17,11 @@
 
 namespace pade {
 
-enum Kind { CD06 = 1, CD10 = 2, CF90 = 3 };
-enum Oper { D1 = 1, D2 = 2, FILTER = 3 };
+enum Kind { CD06 = 1, CD10 = 2, CF90 = 3, CD06STAGG = 4 };
+// cd06stagg (periodic): edge (E) and cell (C) staggered z operators; an E
+// array holds n + 1 planes, the last a copy of the first
+enum Oper { D1 = 1, D2 = 2, FILTER = 3,
+            D1_E2C = 11, D1_C2E = 12, D2_C2C = 13, D2_E2E = 14, INTERP_E2C = 15, INTERP_C2E = 16 };
 
 constexpr int kBandW = 2;      // band half-width handled by the solver
 constexpr int kStencilH = 4;   // max stencil half-width (cf90)

@@ -7,7 +7,7 @@ module padeops_c
     implicit none
     private
 
-    integer(c_int), parameter, public :: PADE_CD06 = 1, PADE_CD10 = 2, PADE_CF90 = 3
+    integer(c_int), parameter, public :: PADE_CD06 = 1, PADE_CD10 = 2, PADE_CF90 = 3, PADE_CD06STAGG = 4
     integer(c_int), parameter, public :: PADE_D1 = 1, PADE_D2 = 2, PADE_FILTER = 3
     integer, parameter, public :: pade_rkind = c_double
     integer, parameter, public :: pade_clen = 256

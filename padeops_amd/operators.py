@@ -22,6 +22,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import PADE_CD06, PADE_CD10, PADE_CF90, PADE_D1, PADE_D2, PADE_FILTER, PadeError
+from ._lib import (PADE_CD06STAGG, PADE_D1_E2C, PADE_D1_C2E, PADE_D2_C2C, PADE_D2_E2E, PADE_INTERP_E2C,
+                   PADE_INTERP_C2E)
 
 try:  # torch is plumbing for device memory and streams
     import torch
@@ -315,3 +317,80 @@ class cf90(_CompactOp):
         _lib.check(lib.pade_op_filter3_complex(self._h, ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(df.data_ptr()),
                                                int(na), int(nb), b1, bn, int(has_bc), _stream_ptr()))
         return df
+
+
+class cd06stagg(_CompactOp):
+    """Sixth-order staggered compact z operators, periodic form
+    (src/derivatives/cd06stagg.F90; init_periodic :170-195).  Cell (C) arrays
+    are (n1, n2, n), edge (E) arrays (n1, n2, n + 1) with the last plane a copy
+    of the first; every operation acts along z.  The non-periodic form
+    (init_nonperiodic, :197) is not provided."""
+
+    _kind = PADE_CD06STAGG
+
+    def init(self, nx, dx, periodic=True):
+        """nx cells; raises PadeError(21) for nx <= 4 (GracefulExit 21,
+        cd06stagg.F90:182-184) or a non-periodic grid."""
+        if not periodic:
+            raise PadeError(21, "the non-periodic staggered operators are not provided")
+        rc = self._create(nx, dx, True, 0, 0)
+        if rc:
+            raise PadeError(rc, "cd06stagg init")
+        return 0
+
+    def _stagg(self, oper, f, df, n1, n2, e_in, e_out):
+        if not self._h:
+            raise PadeError(400, "operator used before init")
+        n = self._n
+        cin, cout = n1 * n2 * (n + e_in), n1 * n2 * (n + e_out)
+        host = isinstance(f, np.ndarray)
+        if host:
+            if f.size != cin or df.size != cout:
+                raise ValueError(f"array sizes must be n1*n2*{n + e_in} and n1*n2*{n + e_out}")
+            lib = _lib.load()
+            fa = np.asfortranarray(f, dtype=np.float64)
+            if df.dtype != np.float64 or not (df.flags.f_contiguous or df.ndim == 1):
+                raise ValueError("df must be a Fortran-ordered float64 array")
+            dp = ctypes.POINTER(ctypes.c_double)
+            _lib.check(lib.pade_op_apply_host(self._h, oper, 3, fa.ctypes.data_as(dp), df.ctypes.data_as(dp),
+                                              int(n1), int(n2), 0, 0))
+            return df
+        if f.dtype != df.dtype or f.dtype not in (torch.float64, torch.float32):
+            raise TypeError("f and df must both be float64 or both float32")
+        if not (f.is_cuda and df.is_cuda):
+            raise TypeError("device arrays must live on the GPU")
+        if f.numel() != cin or df.numel() != cout:
+            raise ValueError(f"array sizes must be n1*n2*{n + e_in} and n1*n2*{n + e_out}")
+        if not (_is_dense_block(f) and _is_dense_block(df)):
+            raise ValueError("arrays must be dense blocks in Fortran order")
+        if f.data_ptr() == df.data_ptr():
+            raise ValueError("f and df must not alias")
+        lib = _lib.load()
+        fn = lib.pade_op_apply if f.dtype == torch.float64 else lib.pade_op_apply_f32
+        _lib.check(fn(self._h, oper, 3, ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(df.data_ptr()),
+                      int(n1), int(n2), 0, 0, _stream_ptr()))
+        return df
+
+    def ddz_E2C(self, fE, dfC, n1, n2):
+        """cd06stagg.F90:820-833."""
+        return self._stagg(PADE_D1_E2C, fE, dfC, n1, n2, 1, 0)
+
+    def ddz_C2E(self, fC, dfE, n1, n2):
+        """cd06stagg.F90:850-864."""
+        return self._stagg(PADE_D1_C2E, fC, dfE, n1, n2, 0, 1)
+
+    def d2dz2_C2C(self, fC, d2fC, n1, n2):
+        """cd06stagg.F90:995-1009."""
+        return self._stagg(PADE_D2_C2C, fC, d2fC, n1, n2, 0, 0)
+
+    def d2dz2_E2E(self, fE, d2fE, n1, n2):
+        """cd06stagg.F90:1027-1042."""
+        return self._stagg(PADE_D2_E2E, fE, d2fE, n1, n2, 1, 1)
+
+    def InterpZ_E2C(self, fE, IfC, n1, n2):
+        """cd06stagg.F90:928-942."""
+        return self._stagg(PADE_INTERP_E2C, fE, IfC, n1, n2, 1, 0)
+
+    def InterpZ_C2E(self, fC, IfE, n1, n2):
+        """cd06stagg.F90:960-975."""
+        return self._stagg(PADE_INTERP_C2E, fC, IfE, n1, n2, 0, 1)

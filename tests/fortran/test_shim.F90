@@ -9,6 +9,7 @@ program test_shim
     use cf90stuff, only: cf90
     use DerivativesMod, only: derivatives
     use FiltersMod, only: filters
+    use cd06staggstuff, only: cd06stagg
     implicit none
     integer, parameter :: rk = kind(0.d0)
     integer, parameter :: n1 = 24, n2 = 20, n3 = 16
@@ -22,6 +23,9 @@ program test_shim
     type(cf90) :: c90
     type(derivatives) :: der
     type(filters) :: fil
+    type(cd06stagg) :: stg
+    real(rk), allocatable :: fe(:, :, :), ge(:, :, :)
+    complex(rk), allocatable :: fce(:, :, :), gcc(:, :, :)
 
     call get_command_argument(1, path)
     if (trim(path) == "badbc") then
@@ -113,6 +117,22 @@ program test_shim
     call fil%init(n1, n2, n3, .true., .true., .false., "cf90", "cf90", "cf90")
     call fil%filterz(f, g)
     call put("fil_filterz", g)
+
+    ! staggered z operators, periodic (cd06stagg.F90): E arrays of n3 + 1 planes
+    allocate(fe(n1, n2, n3 + 1), ge(n1, n2, n3 + 1), fce(n1, n2, n3 + 1), gcc(n1, n2, n3))
+    fe(:, :, 1:n3) = f
+    fe(:, :, n3 + 1) = f(:, :, 1)
+    call stg%init(n3, 2 * pi / n3)
+    call stg%ddz_E2C(fe, g, n1, n2)
+    call put("stg_d1_e2c", g)
+    call stg%InterpZ_C2E(f, ge, n1, n2)
+    call put("stg_interp_c2e", ge)
+    fce(:, :, 1:n3) = cmplx(f, 2.0_rk * f, kind=rk)
+    fce(:, :, n3 + 1) = fce(:, :, 1)
+    call stg%InterpZ_E2C(fce, gcc, n1, n2)
+    call put("stg_ie2c_re", real(gcc, kind=rk))
+    call put("stg_ie2c_im", aimag(gcc))
+    call stg%destroy()
 
     ! init error codes (the reference's ierr values)
     ierr = c10%init(4, 0.1_rk, .true., 0, 0)

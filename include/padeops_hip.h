@@ -35,22 +35,30 @@ extern "C" {
 #endif
 
 enum pade_kind { PADE_CD06 = 1, PADE_CD10 = 2, PADE_CF90 = 3, PADE_CD06STAGG = 4 };
-/* PADE_CD06STAGG (periodic only; cd06stagg.F90:170-195 init_periodic, n > 4)
- * takes the staggered operations, along z (axis 3) only, on edge (E) arrays of
- * n + 1 planes (the last a copy of the first) and cell (C) arrays of n:
+/* PADE_CD06STAGG (n > 4) takes the staggered operations, along z (axis 3)
+ * only, on edge (E) arrays of n + 1 planes and cell (C) arrays of n:
  *   PADE_D1_E2C      ddz_E2C      (cd06stagg.F90:820)
  *   PADE_D1_C2E      ddz_C2E      (:850)
  *   PADE_D2_C2C      d2dz2_C2C    (:995)
  *   PADE_D2_E2E      d2dz2_E2E    (:1027)
  *   PADE_INTERP_E2C  InterpZ_E2C  (:928)
  *   PADE_INTERP_C2E  InterpZ_C2E  (:960)
- * through pade_op_apply / _f32 / _axpby (f and df sized by their grid);
- * E outputs get their last plane copied from the first, as the reference
- * does.  No adjoint and no dense form. */
+ *   PADE_D1_C2C      ddz_C2C      (:883, bounded only)
+ *   PADE_D1_E2E      ddz_E2E      (:905, bounded only)
+ * through pade_op_apply / _f32 / _axpby (f and df sized by their grid).
+ * Periodic (init_periodic, cd06stagg.F90:170-195): an E array's last plane
+ * repeats its first, and E outputs get it copied, as the reference does.
+ * Bounded (init_nonperiodic, cd06stagg.F90:197-232): create with
+ * periodic = 0 and bc1 (the k = 1 end, "Bot") / bcn (the k = n end, "Top")
+ * each +1 (isEven), -1 (odd), +2 (isSided, even) or -2 (isSided, odd); the
+ * apply-time bc1 / bcn are ignored.  The first derivatives E2C, C2E, C2C
+ * and both interpolations have one-sided closures; ddz_E2E and the second
+ * derivatives use the parity only, as in the reference.  No adjoint and no
+ * dense form. */
 enum pade_oper {
     PADE_D1 = 1, PADE_D2 = 2, PADE_FILTER = 3,
     PADE_D1_E2C = 11, PADE_D1_C2E = 12, PADE_D2_C2C = 13, PADE_D2_E2E = 14,
-    PADE_INTERP_E2C = 15, PADE_INTERP_C2E = 16
+    PADE_INTERP_E2C = 15, PADE_INTERP_C2E = 16, PADE_D1_C2C = 17, PADE_D1_E2E = 18
 };
 enum pade_dense_prec { PADE_DENSE_F32 = 1, PADE_DENSE_BF16X3 = 2, PADE_DENSE_FULL = 16 };
 enum pade_dtype { PADE_F64 = 0, PADE_F32 = 1 };
@@ -61,8 +69,10 @@ typedef struct pade_op_s *pade_op_t;
  * Replaces cd06%init (src/derivatives/cd06.F90:129), cd10%init
  * (src/derivatives/cd10.F90:195) and, with dx ignored, cf90%init
  * (src/filters/cf90.F90:107).  bc1/bcn matter only for bounded cd06 (they
- * select its LHS closure at init, cd06.F90:264-327); cd10 / cf90 take the bc
- * at apply time.  Returns the reference's ierr (2, 3, 7) for too small n. */
+ * select its LHS closure at init, cd06.F90:264-327) and bounded cd06stagg
+ * (the end codes listed with the staggered operations); cd10 / cf90 take the
+ * bc at apply time.  Returns the reference's ierr (2, 3, 7; 21 for cd06stagg
+ * with n <= 4) for too small n. */
 int pade_op_create(int kind, int n, double dx, int periodic, int bc1, int bcn, pade_op_t *op);
 
 /* Release device tables.  Replaces cd06/cd10/cf90 %destroy
@@ -164,9 +174,12 @@ int pade_permute3(const void *src, void *dst, int elem_size, int n1, int n2, int
 /* Host-only planning export (no GPU needed): writes the FP64 table the
  * kernels consume for (kind, oper, n, dx, periodic, bc1, bcn, adjoint) with
  * partition size M (M = 0: library choice), and meta[32] =
- * {n, M, P, periodic, nbl, nbr, off_bl, off_br, off_coef, coef_len, ncls, kq,
- *  len, 0, 0, 0, cls[0..15]}: stencil at [0,9), left/right boundary rows
- * (8 x 16) at off_bl/off_br, then at off_coef ncls blocks of 10*M values
+ * {rows, M, P, periodic, nbl, nbr, off_bl, off_br, off_coef, coef_len, ncls,
+ *  kq, len, cols, 0, 0, cls[0..15]}: rows = output points per pencil, cols =
+ * input points (both n except for the staggered cell <-> edge operators,
+ * which take M = rows); stencil at [0,9), left/right boundary rows (8 x 16,
+ * the right ones reading input point cols - 1 - c) at off_bl/off_br, then at
+ * off_coef ncls blocks of 10*M values
  * (M band-LU rows l1, l2, 1/pivot, u1/pivot, u2/pivot, 0 and M coupling rows
  * of 4) followed by the reduced-system rows (periodic: the 4 rows of 4P values
  * of partition 0, which partition p uses rotated by p; bounded: 4 rows for

@@ -546,3 +546,407 @@ def stagg_apply(op, f, dx):
     if STAGG_EDGE_OUT[op]:
         out = np.concatenate([out, out[:, :, :1]], axis=2)
     return np.asfortranarray(out)
+
+
+# ----------------------------------------------------------------------------
+# Bounded staggered operators (cd06stagg.F90 init_nonperiodic :197-232)
+# ----------------------------------------------------------------------------
+#
+# Restated statement by statement from the reference's right-hand-side files
+# (STAGG_CD06_files/*RHS*_common.F90) and tridiagonal set-up
+# (STAGG_CD06_files/ComputeTri_allRoutines.F90), in their order of
+# assignment, with the reference's 1-based indices kept in the helpers below.
+# The solve is a dense LAPACK solve of the (ddn, dg, dup) tridiagonal matrix,
+# so the reference's Thomas sweeps (TridiagSolver_allRoutines.F90) are not
+# restated.  The ends are named as the reference does: "bot" is the k = 1
+# end, "top" the k = n end; each is (even, sided).
+
+STAGG_OPS_BOUNDED = ("d1_e2c", "d1_c2e", "d1_c2c", "d1_e2e", "d2_c2c", "d2_e2e",
+                     "interp_e2c", "interp_c2e")
+STAGG_EDGE_IN.update({"d1_c2c": False, "d1_e2e": True})
+STAGG_EDGE_OUT.update({"d1_c2c": False, "d1_e2e": True})
+
+# module parameters, cd06stagg.F90:18-56
+_SG_W0S, _SG_W1S = 223.0 / 186.0, 61.0 / 62.0
+
+
+def _stagg_c2c_closure():
+    """Lele's one-sided closure of the cell first derivative, cd06stagg.F90:18-53."""
+    p, q, r, s = 17.0 / 6.0, 1.5, 1.5, -1.0 / 6.0
+    ah = 1.0 / 3.0
+    q_p, alpha_p = 0.75, 0.25
+    alpha_pp = (((40 * ah - 1) * q + 7 * (4 * ah - 1) * s)
+                / (16 * (ah + 2) * q + 8 * (1 - 4 * ah) * s))
+    q_pp = (alpha_pp + 2) / 3.0
+    r_pp = (4 * alpha_pp - 1) / 12.0
+    w1 = (2 * ah + 1) / (2 * (q + s))
+    w2 = ((8 * ah + 7) * q - 6 * (2 * ah + 1) * r + (8 * ah + 7) * s) / (9 * (q + s))
+    w3 = (4 * (ah + 2) * q + 2 * (1 - 4 * ah) * s) / (9 * (q + s))
+    return dict(p=p, q=q, r=r, s=s, q_p=q_p, alpha_p=alpha_p, alpha_pp=alpha_pp,
+                q_pp=q_pp, r_pp=r_pp, w1=w1, w2=w2, w3=w3)
+
+
+class _Rows:
+    """Dense rows addressed with the reference's 1-based indices."""
+
+    def __init__(self, m, ncol):
+        self.M = np.zeros((m, ncol))
+
+    def set(self, i, terms):
+        self.M[i - 1, :] = 0.0
+        self.add(i, terms)
+
+    def add(self, i, terms):
+        for c, v in terms:
+            self.M[i - 1, c - 1] += v
+
+
+def _stagg_rhs(op, n, dx, bot, top):
+    nE = n + 1
+    bot_even, bot_sided = bot
+    top_even, top_sided = top
+    W0S, W1S = _SG_W0S, _SG_W1S
+    if op == "d1_e2c":                  # D1RHS_E2C_common.F90:1-40
+        a, b = 63.0 / 62.0, (17.0 / 62.0) / 3.0
+        alpha1, alpha0 = 37.0 / 183.0, -1.0
+        a0 = (alpha0 - 23.0) / 24.0
+        b0 = (-9.0 * alpha0 + 7.0) / 8.0
+        c0 = (9.0 * alpha0 + 1.0) / 8.0
+        d0 = -(alpha0 + 1.0) / 24.0
+        a1 = (3.0 / 8.0) * (3.0 - 2.0 * alpha1)
+        b1 = (1.0 / 8.0) * (-1.0 + 22.0 * alpha1)
+        R = _Rows(n, nE)
+        for i in range(2, n):
+            R.set(i, [(i + 2, b), (i - 1, -b), (i + 1, a), (i, -a)])
+        if bot_sided:
+            R.set(1, [(1, W0S * a0), (2, W0S * b0), (3, W0S * c0), (4, W0S * d0)])
+            R.set(2, [(1, -W1S * b1 / 3), (2, -W1S * a1), (3, W1S * a1), (4, W1S * b1 / 3)])
+        elif bot_even:
+            R.set(1, [(3, b), (2, -b), (2, a), (1, -a)])
+        else:
+            R.set(1, [(3, b), (2, b), (2, a), (1, -a)])
+        if top_sided:
+            R.set(n, [(nE, -W0S * a0), (nE - 1, -W0S * b0), (nE - 2, -W0S * c0), (nE - 3, -W0S * d0)])
+            R.set(n - 1, [(nE - 3, -W1S * b1 / 3), (nE - 2, -W1S * a1), (nE - 1, W1S * a1),
+                          (nE, W1S * b1 / 3)])
+        elif top_even:
+            R.set(n, [(nE - 1, b), (nE - 2, -b), (nE, a), (nE - 1, -a)])
+        else:
+            R.set(n, [(nE - 1, -b), (nE - 2, -b), (nE, a), (nE - 1, -a)])
+        return R.M / dx
+    if op == "d1_c2e":                  # D1RHS_C2E_common.F90:1-36
+        a, b = 63.0 / 62.0, (17.0 / 62.0) / 3.0
+        a0, b0, c0, d0 = -71.0 / 24.0, 47.0 / 8.0, -31.0 / 8.0, 23.0 / 24.0
+        a1 = 12.0 / 11.0
+        R = _Rows(nE, n)
+        for i in range(2, nE):
+            R.set(i, [(i, a), (i - 1, -a)])
+        for i in range(3, nE - 1):
+            R.add(i, [(i + 1, b), (i - 2, -b)])
+        if bot_sided:
+            R.set(1, [(1, a0), (2, b0), (3, c0), (4, d0)])
+            R.set(2, [(2, a1), (1, -a1)])
+        elif bot_even:
+            R.set(1, [])
+            R.add(2, [(3, b), (1, -b)])
+        else:
+            R.set(1, [(2, 2 * b), (1, 2 * a)])
+            R.add(2, [(3, b), (1, b)])
+        if top_sided:
+            R.set(nE - 1, [(n, a1), (n - 1, -a1)])
+            R.set(nE, [(n, -a0), (n - 1, -b0), (n - 2, -c0), (n - 3, -d0)])
+        elif top_even:
+            R.add(nE - 1, [(n, b), (n - 2, -b)])
+            R.set(nE, [])
+        else:
+            R.add(nE - 1, [(n, -b), (n - 2, -b)])
+            R.set(nE, [(n - 1, -2 * b), (n, -2 * a)])
+        return R.M / dx
+    if op == "d1_c2c":                  # D1RHS_C2C_common.F90:1-71
+        a, b = (14.0 / 9.0) / 2.0, (1.0 / 9.0) / 4.0
+        k = _stagg_c2c_closure()
+        a_np_3, b_np_3 = k["w3"] * k["q_pp"], k["w3"] * k["r_pp"]
+        a_np_2 = k["w2"] * k["q_p"]
+        a_np_1, b_np_1 = k["w1"] * (-k["p"]), k["w1"] * k["q"]
+        c_np_1, d_np_1 = k["w1"] * k["r"], k["w1"] * k["s"]
+        R = _Rows(n, n)
+        if bot_sided:
+            R.set(1, [(1, a_np_1), (2, b_np_1), (3, c_np_1), (4, d_np_1)])
+            R.set(2, [(3, a_np_2), (1, -a_np_2)])
+            R.set(3, [(4, a_np_3), (2, -a_np_3), (5, b_np_3), (1, -b_np_3)])
+        elif bot_even:
+            R.set(1, [(3, b), (2, -b), (2, a), (1, -a)])
+            R.set(2, [(4, b), (1, -b), (3, a), (1, -a)])
+            R.set(3, [(5, b), (1, -b), (4, a), (2, -a)])
+        else:
+            R.set(1, [(3, b), (2, b), (2, a), (1, a)])
+            R.set(2, [(4, b), (1, b), (3, a), (1, -a)])
+            R.set(3, [(5, b), (1, -b), (4, a), (2, -a)])
+        for i in range(4, n - 2):
+            R.set(i, [(i + 2, b), (i - 2, -b), (i + 1, a), (i - 1, -a)])
+        if top_sided:
+            R.set(n - 2, [(n - 1, a_np_3), (n - 3, -a_np_3), (n, b_np_3), (n - 4, -b_np_3)])
+            R.set(n - 1, [(n, a_np_2), (n - 2, -a_np_2)])
+            R.set(n, [(n, -a_np_1), (n - 1, -b_np_1), (n - 2, -c_np_1), (n - 3, -d_np_1)])
+        elif top_even:
+            R.set(n - 2, [(n, b), (n - 4, -b), (n - 1, a), (n - 3, -a)])
+            R.set(n - 1, [(n, b), (n - 3, -b), (n, a), (n - 2, -a)])
+            R.set(n, [(n - 1, b), (n - 2, -b), (n, a), (n - 1, -a)])
+        else:
+            R.set(n - 2, [(n, b), (n - 4, -b), (n - 1, a), (n - 3, -a)])
+            R.set(n - 1, [(n, -b), (n - 3, -b), (n, a), (n - 2, -a)])
+            R.set(n, [(n - 1, -b), (n - 2, -b), (n, -a), (n - 1, -a)])
+        return R.M / dx
+    if op == "d1_e2e":                  # D1RHS_E2E_common.F90:1-22
+        a, b = (14.0 / 9.0) / 2.0, (1.0 / 9.0) / 4.0
+        R = _Rows(nE, nE)
+        for i in range(2, nE):
+            R.set(i, [(i + 1, a), (i - 1, -a)])
+        for i in range(3, nE - 1):
+            R.add(i, [(i + 2, b), (i - 2, -b)])
+        if bot_even:
+            R.set(1, [])
+            R.add(2, [(4, b), (2, -b)])
+        else:
+            R.set(1, [(3, 2 * b), (2, 2 * a)])
+            R.add(2, [(4, b), (2, b)])
+        if top_even:
+            R.set(nE, [])
+            R.add(nE - 1, [(nE - 1, b), (nE - 3, -b)])
+        else:
+            R.set(nE, [(nE - 2, -2 * b), (nE - 1, -2 * a)])
+            R.add(nE - 1, [(nE - 1, -b), (nE - 3, -b)])
+        return R.M / dx
+    if op in ("d2_c2c", "d2_e2e"):      # D2RHS_C2C_common.F90:1-26, D2RHS_E2E_common.F90:1-25
+        a, b = 12.0 / 11.0, (3.0 / 11.0) / 4.0
+        m = n if op == "d2_c2c" else nE
+        R = _Rows(m, m)
+        for i in range(2, m):
+            R.set(i, [(i + 1, a), (i - 1, a)])
+        for i in range(3, m - 1):
+            R.add(i, [(i + 2, b), (i - 2, b)])
+        if op == "d2_c2c":
+            if bot_even:
+                R.set(1, [(3, b), (2, b), (2, a), (1, a)])
+                R.add(2, [(4, b), (1, b)])
+            else:
+                R.set(1, [(3, b), (2, -b), (2, a), (1, -a)])
+                R.add(2, [(4, b), (1, -b)])
+            if top_even:
+                R.set(m, [(m - 1, b), (m - 2, b), (m, a), (m - 1, a)])
+                R.add(m - 1, [(m, b), (m - 3, b)])
+            else:
+                R.set(m, [(m - 1, -b), (m - 2, b), (m, -a), (m - 1, a)])
+                R.add(m - 1, [(m, -b), (m - 3, b)])
+        else:
+            if bot_even:
+                R.set(1, [(3, 2 * b), (2, 2 * a)])
+                R.add(2, [(4, b), (2, b)])
+            else:
+                R.set(1, [])
+                R.add(2, [(4, b), (2, -b)])
+            if top_even:
+                R.set(m, [(m - 2, 2 * b), (m - 1, 2 * a)])
+                R.add(m - 1, [(m - 1, b), (m - 3, b)])
+            else:
+                R.set(m, [])
+                R.add(m - 1, [(m - 1, -b), (m - 3, b)])
+        R.M -= 2.0 * (a + b) * np.eye(m)
+        return R.M / dx ** 2
+    if op == "interp_c2e":              # InterpRHS_C2E_common.F90:1-32
+        b, a = 0.1 / 2.0, 1.5 / 2.0
+        a0, b0, c0 = 15.0 / 8.0, -5.0 / 4.0, 3.0 / 8.0
+        alpha1 = 1.0 / 6.0
+        a1 = (9.0 + 10.0 * alpha1) / 8.0
+        R = _Rows(nE, n)
+        for i in range(2, nE):
+            R.set(i, [(i, a), (i - 1, a)])
+        for i in range(3, nE - 1):
+            R.add(i, [(i + 1, b), (i - 2, b)])
+        if bot_sided:
+            R.set(1, [(1, a0), (2, b0), (3, c0)])
+            R.set(2, [(2, a1 / 2), (1, a1 / 2)])
+        elif bot_even:
+            R.set(1, [(2, 2 * b), (1, 2 * a)])
+            R.add(2, [(3, b), (1, b)])
+        else:
+            R.set(1, [])
+            R.add(2, [(3, b), (1, -b)])
+        if top_sided:
+            R.set(nE, [(n, a0), (n - 1, b0), (n - 2, c0)])
+            R.set(nE - 1, [(n, a1 / 2), (n - 1, a1 / 2)])
+        elif top_even:
+            R.set(nE, [(n - 1, 2 * b), (n, 2 * a)])
+            R.add(nE - 1, [(n, b), (n - 2, b)])
+        else:
+            R.set(nE, [])
+            R.add(nE - 1, [(n, -b), (n - 2, b)])
+        return R.M
+    if op == "interp_e2c":              # InterpRHS_E2C_common.F90:1-26
+        b, a = 0.1 / 2.0, 1.5 / 2.0
+        alpha0 = 1.0
+        a0 = (5.0 - alpha0) / 16.0
+        b0 = (9.0 * alpha0 + 15.0) / 16.0
+        c0 = (9.0 * alpha0 - 5.0) / 16.0
+        d0 = (1.0 - alpha0) / 16.0
+        R = _Rows(n, nE)
+        if bot_sided:
+            R.set(1, [(1, a0), (2, b0), (3, c0), (4, d0)])
+        elif bot_even:
+            R.set(1, [(3, b), (2, b), (2, a), (1, a)])
+        else:
+            R.set(1, [(3, b), (2, -b), (2, a), (1, a)])
+        for i in range(2, n):
+            R.set(i, [(i + 2, b), (i - 1, b), (i + 1, a), (i, a)])
+        if top_sided:
+            R.set(n, [(nE, a0), (nE - 1, b0), (nE - 2, c0), (nE - 3, d0)])
+        elif top_even:
+            R.set(n, [(nE - 1, b), (nE - 2, b), (nE, a), (nE - 1, a)])
+        else:
+            R.set(n, [(nE - 1, -b), (nE - 2, b), (nE, a), (nE - 1, a)])
+        return R.M
+    raise ValueError(op)
+
+
+def _stagg_lhs(op, n, bot, top):
+    """(ddn, dg, dup) of ComputeTri_allRoutines.F90 as a dense matrix."""
+    nE = n + 1
+    bot_even, bot_sided = bot
+    top_even, top_sided = top
+    W0S, W1S = _SG_W0S, _SG_W1S
+    alpha = {"d1_e2c": 9.0 / 62.0, "d1_c2e": 9.0 / 62.0, "d1_c2c": 1.0 / 3.0,
+             "d1_e2e": 1.0 / 3.0, "d2_c2c": 2.0 / 11.0, "d2_e2e": 2.0 / 11.0,
+             "interp_e2c": 3.0 / 10.0, "interp_c2e": 3.0 / 10.0}[op]
+    m = n if op in ("d1_e2c", "d1_c2c", "d2_c2c", "interp_e2c") else nE
+    ddn = np.full(m + 1, alpha)     # 1-based: element 0 unused
+    dg = np.ones(m + 1)
+    dup = np.full(m + 1, alpha)
+    if op == "d1_e2c":                  # ComputeTri_allRoutines.F90:1-48
+        alpha1, alpha0 = 37.0 / 183.0, -1.0
+        if top_sided:
+            ddn[m], dg[m] = W0S * alpha0, W0S
+            ddn[m - 1], dup[m - 1], dg[m - 1] = W1S * alpha1, W1S * alpha1, W1S
+        elif top_even:
+            dup[m], dg[m] = 0.0, 1.0 - alpha
+        else:
+            dg[m], dup[m] = 1.0 + alpha, 0.0
+        if bot_sided:
+            dup[1], dg[1] = W0S * alpha0, W0S
+            ddn[2], dup[2], dg[2] = W1S * alpha1, W1S * alpha1, W1S
+        elif bot_even:
+            ddn[1], dg[1] = 0.0, 1.0 - alpha
+        else:
+            ddn[1], dg[1] = 0.0, 1.0 + alpha
+    elif op == "d1_c2e":                # :50-96
+        if top_sided:
+            ddn[m], dup[m] = 0.0, 0.0
+            ddn[m - 1], dup[m - 1] = 1.0 / 22.0, 1.0 / 22.0
+        elif top_even:
+            dup[m], ddn[m] = 0.0, 0.0
+        else:
+            ddn[m], dup[m] = 2 * alpha, 0.0
+        if bot_sided:
+            dup[1], ddn[1] = 0.0, 0.0
+            dup[2], ddn[2] = 1.0 / 22.0, 1.0 / 22.0
+        elif bot_even:
+            ddn[1], dup[1] = 0.0, 0.0
+        else:
+            ddn[1], dup[1] = 0.0, 2 * alpha
+    elif op == "d1_c2c":                # :98-168
+        k = _stagg_c2c_closure()
+        alpha_low = 3.0
+        if top_sided:
+            dup[m], dup[m - 1], dup[m - 2] = 0.0, k["w2"] * k["alpha_p"], k["w3"] * k["alpha_pp"]
+            dg[m], dg[m - 1], dg[m - 2] = k["w1"], k["w2"], k["w3"]
+            ddn[m], ddn[m - 1], ddn[m - 2] = (k["w1"] * alpha_low, k["w2"] * k["alpha_p"],
+                                              k["w3"] * k["alpha_pp"])
+        elif top_even:
+            dg[m] = 1.0 - alpha
+        else:
+            dg[m] = 1.0 + alpha
+        if bot_sided:
+            ddn[1], ddn[2], ddn[3] = 0.0, k["w2"] * k["alpha_p"], k["w3"] * k["alpha_pp"]
+            dg[1], dg[2], dg[3] = k["w1"], k["w2"], k["w3"]
+            dup[1], dup[2], dup[3] = (k["w1"] * alpha_low, k["w2"] * k["alpha_p"],
+                                      k["w3"] * k["alpha_pp"])
+        elif bot_even:
+            dg[1] = 1.0 - alpha
+        else:
+            dg[1] = 1.0 + alpha
+    elif op == "d1_e2e":                # :170-208
+        if top_even:
+            dg[m], ddn[m] = 1.0, 0.0
+        else:
+            dg[m], ddn[m] = 1.0, 2 * alpha
+        if bot_even:
+            dg[1], dup[1] = 1.0, 0.0
+        else:
+            dg[1], dup[1] = 1.0, 2 * alpha
+    elif op == "interp_c2e":            # :210-255
+        alpha1 = 1.0 / 6.0
+        if top_sided:
+            dup[m], ddn[m] = 0.0, 0.0
+            dup[m - 1], ddn[m - 1] = alpha1, alpha1
+        elif top_even:
+            ddn[m], dg[m] = 2 * alpha, 1.0
+        else:
+            ddn[m], dg[m] = 0.0, 1.0
+        if bot_sided:
+            dup[1], ddn[1], dg[1] = 0.0, 0.0, 1.0
+            dup[2], ddn[2], dg[2] = alpha1, alpha1, 1.0
+        elif bot_even:
+            dup[1], dg[1] = 2 * alpha, 1.0
+        else:
+            dup[1], dg[1] = 0.0, 1.0
+    elif op == "interp_e2c":            # :258-299
+        alpha0 = 1.0
+        if top_sided:
+            ddn[m], dup[m] = alpha0, 0.0
+        elif top_even:
+            dg[m] = 1.0 + alpha
+        else:
+            dg[m] = 1.0 - alpha
+        if bot_sided:
+            dup[1], ddn[1] = alpha0, 0.0
+        elif bot_even:
+            dg[1] = 1.0 + alpha
+        else:
+            dg[1] = 1.0 - alpha
+    elif op == "d2_e2e":                # :302-335
+        if top_even:
+            dg[m], ddn[m] = 1.0, 2 * alpha
+        else:
+            dg[m], ddn[m] = 1.0, 0.0
+        if bot_even:
+            dg[1], dup[1] = 1.0, 2 * alpha
+        else:
+            dg[1], dup[1] = 1.0, 0.0
+    elif op == "d2_c2c":                # :337-366
+        dg[m] = 1.0 + alpha if top_even else 1.0 - alpha
+        dg[1] = 1.0 + alpha if bot_even else 1.0 - alpha
+    else:
+        raise ValueError(op)
+    A = np.diag(dg[1:])
+    for i in range(2, m + 1):
+        A[i - 1, i - 2] = ddn[i]
+    for i in range(1, m):
+        A[i - 1, i] = dup[i]
+    return A
+
+
+def stagg_bounded_matrices(op, n, dx, bot, top):
+    """(A, B) of a bounded staggered operation: A (m x m), B (m x n_in)."""
+    return _stagg_lhs(op, n, bot, top), _stagg_rhs(op, n, dx, bot, top)
+
+
+def stagg_bounded_operator(op, n, dx, bot, top):
+    """D = A^{-1} B (m x n_in) of a bounded staggered operation."""
+    A, B = stagg_bounded_matrices(op, n, dx, bot, top)
+    return np.linalg.solve(A, B)
+
+
+def stagg_bounded_apply(op, f, dx, bot, top):
+    """Apply along z (last axis) of a Fortran-ordered (n1, n2, n or n+1) array."""
+    n = f.shape[2] - (1 if STAGG_EDGE_IN[op] else 0)
+    D = stagg_bounded_operator(op, n, dx, bot, top)
+    return np.asfortranarray(np.einsum("ij,abj->abi", D, f))

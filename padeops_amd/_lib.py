@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libpadeops_hip.so")
 PADE_CD06, PADE_CD10, PADE_CF90, PADE_CD06STAGG = 1, 2, 3, 4
 PADE_D1, PADE_D2, PADE_FILTER = 1, 2, 3
 PADE_D1_E2C, PADE_D1_C2E, PADE_D2_C2C, PADE_D2_E2E, PADE_INTERP_E2C, PADE_INTERP_C2E = 11, 12, 13, 14, 15, 16
+PADE_D1_C2C, PADE_D1_E2E = 17, 18
 PADE_DENSE_F32, PADE_DENSE_BF16X3, PADE_DENSE_FULL = 1, 2, 16
 PADE_F64, PADE_F32 = 0, 1
 

@@ -145,16 +145,14 @@ int operator_matrices(int kind, int oper, int n, double dx, bool periodic, int b
 bool oper_valid(int kind, int oper)
 {
     if (kind == pade::CF90) return oper == pade::FILTER;
-    if (kind == pade::CD06STAGG) return oper >= pade::D1_E2C && oper <= pade::INTERP_C2E;
+    if (kind == pade::CD06STAGG) return oper >= pade::D1_E2C && oper <= pade::D1_E2E;
     return oper == pade::D1 || oper == pade::D2;
 }
 
-// Staggered operations whose output lives on the n + 1 edge planes (the
-// last a copy of the first, cd06stagg.F90:859,969,1036).
-bool edge_output(int kind, int oper)
-{
-    return kind == pade::CD06STAGG && (oper == pade::D1_C2E || oper == pade::D2_E2E || oper == pade::INTERP_C2E);
-}
+// Staggered operations whose input / output lives on the n + 1 edge planes
+// (periodic: the last a copy of the first, cd06stagg.F90:859,969,1036).
+bool edge_output(int kind, int oper) { return kind == pade::CD06STAGG && pade::stagg_edge_out(oper); }
+bool edge_input(int kind, int oper) { return kind == pade::CD06STAGG && pade::stagg_edge_in(oper); }
 
 }  // namespace
 
@@ -174,7 +172,10 @@ struct pade_op_s {
 
 namespace {
 
-int plan_key_bc(const pade_op_s *op, int bc) { return op->kind == pade::CD06 ? 0 : bc; }
+int plan_key_bc(const pade_op_s *op, int bc)
+{
+    return op->kind == pade::CD06 || op->kind == pade::CD06STAGG ? 0 : bc;
+}
 
 // Plan key of the complex z filter's mixed operator (bounded cf90): the
 // right-hand side of boundary codes (bc1, bcn) solved with the (0, 0)
@@ -188,11 +189,14 @@ int build_all_plans(pade_op_s *op)
     std::vector<double> tab;
     std::vector<int> opers;
     if (op->kind == pade::CF90) opers = {pade::FILTER};
-    else if (op->kind == pade::CD06STAGG)
+    else if (op->kind == pade::CD06STAGG) {
         opers = {pade::D1_E2C, pade::D1_C2E, pade::D2_C2C, pade::D2_E2E, pade::INTERP_E2C, pade::INTERP_C2E};
-    else opers = {pade::D1, pade::D2};
+        if (!op->periodic) opers.insert(opers.end(), {pade::D1_C2C, pade::D1_E2E});
+    } else opers = {pade::D1, pade::D2};
+    // init-time end codes choose cd06's and the staggered operators' closures
+    const bool init_bc = op->kind == pade::CD06 || op->kind == pade::CD06STAGG;
     std::vector<std::pair<int, int>> bcs;
-    if (op->periodic || op->kind == pade::CD06) bcs = {{0, 0}};
+    if (op->periodic || init_bc) bcs = {{0, 0}};
     else
         for (int a = -1; a <= 1; a++)
             for (int b = -1; b <= 1; b++) bcs.push_back({a, b});
@@ -202,18 +206,23 @@ int build_all_plans(pade_op_s *op)
             for (int adj = 0; adj < 2; adj++) {
                 if (adj && op->n < 16) continue;  // adjoint needs n >= 16 (boundary rows of B^T)
                 if (adj && op->kind == pade::CD06STAGG) continue;   // no staggered adjoint
-                const int cb1 = op->kind == pade::CD06 ? op->init_bc1 : b1;
-                const int cbn = op->kind == pade::CD06 ? op->init_bcn : bn;
+                const int cb1 = init_bc ? op->init_bc1 : b1;
+                const int cbn = init_bc ? op->init_bcn : bn;
                 Plan pl;
                 if (int rc = operator_matrices(op->kind, oper, op->n, op->dx, op->periodic, cb1, cbn,
                                                adj != 0, pl.mats))
                     return fail(rc, "operator construction failed");
+                // rows of this operator (a staggered cell <-> edge operator has
+                // one more or one fewer than n); the partitioned kernels need a
+                // square B
+                const int m = pl.mats.n;
+                const bool square = pl.mats.n_in == 0 || pl.mats.n_in == m;
                 for (int M : {16, 32, 64}) {
-                    if (op->n % M || op->n / M > pade::kMaxPartitions) continue;
+                    if (!square || m % M || m / M > pade::kMaxPartitions) continue;
                     Variant v;
                     if (build_variant(pl.mats, M, tab, v) == 0) pl.spike.push_back(v);
                 }
-                if (op->n >= 4 && build_variant(pl.mats, op->n, tab, pl.generic) == 0) pl.has_generic = true;
+                if (m >= 4 && build_variant(pl.mats, m, tab, pl.generic) == 0) pl.has_generic = true;
                 op->plans[{oper, b1, bn, adj}] = std::move(pl);
             }
     }
@@ -324,11 +333,12 @@ template <typename T>
 int apply_one(pade_op_t op, const Plan &pl, int axis, const T *f, T *df, int na, int nb, bool adj,
               hipStream_t s, Epilogue<T> epi)
 {
-    const int n = op->n;
+    const int n = pl.mats.n;
     pade::ApplyArgs<T> a{};
     a.f = f;
     a.out = df;
     a.n = n;
+    a.n_in = pl.mats.n_in ? pl.mats.n_in : n;
     a.periodic = op->periodic ? 1 : 0;
     a.npencils = (long long)na * nb;
     int mode = 0;
@@ -412,6 +422,8 @@ int apply_impl(pade_op_t op, int oper, int axis, const T *f, T *df, int na, int 
     if (op->kind == pade::CD06STAGG) {
         if (axis != 3) return fail(400, "the staggered operators act along z (axis 3)");
         if (adj) return fail(21, "no adjoint of the staggered operators");
+        if (op->periodic && (oper == pade::D1_C2C || oper == pade::D1_E2E))
+            return fail(21, "ddz_C2C and ddz_E2E are provided for the bounded staggered operators only");
     }
     if (op->kind != pade::CD06) {
         auto ok = [](int b) { return b == 0 || b == 1 || b == -1; };
@@ -447,7 +459,7 @@ int apply_impl(pade_op_t op, int oper, int axis, const T *f, T *df, int na, int 
     if (it == op->plans.end())
         return fail(21, adj ? "adjoint needs n >= 16" : "no plan for this operation");
     const int rc = apply_one<T>(op, it->second, axis, f, df, na, nb, adj, s, epi);
-    if (!rc && edge_output(op->kind, oper)) {
+    if (!rc && op->periodic && edge_output(op->kind, oper)) {
         // an edge array's last plane repeats its first (cd06stagg.F90:859,969,1036)
         const size_t plane = (size_t)na * nb;
         hipError_t e = hipMemcpyAsync(df + plane * op->n, df, sizeof(T) * plane, hipMemcpyDeviceToDevice, s);
@@ -498,8 +510,12 @@ int pade_op_create(int kind, int n, double dx, int periodic, int bc1, int bcn, p
     if (kind != pade::CF90 && !(dx > 0.0) && n > 1) return fail(400, "dx must be positive");
     if (int ierr = pade::init_error(kind, n, periodic != 0))
         return fail(ierr, "grid too small for this operator");
-    if (kind == pade::CD06STAGG && !periodic)
-        return fail(21, "the non-periodic staggered operators are not provided");
+    if (kind == pade::CD06STAGG && !periodic) {
+        auto ok = [](int b) { return b == 1 || b == -1 || b == 2 || b == -2; };
+        if (!ok(bc1) || !ok(bcn))
+            return fail(400, "bounded staggered ends: bc1 / bcn must be +-1 (even / odd) or "
+                             "+-2 (one-sided closure, even / odd)");
+    }
     auto op = std::make_unique<pade_op_s>();
     op->kind = kind;
     op->n = n;
@@ -583,8 +599,7 @@ int pade_op_apply_host(pade_op_t op, int oper, int axis, const double *f, double
     if (!f || !df) return fail(400, "null array");
     if (na < 1 || nb < 1) return fail(400, "na and nb must be positive");
     // staggered edge arrays hold one plane more than the pencil length
-    const bool stagg = op->kind == pade::CD06STAGG;
-    const bool e_in = stagg && (oper == pade::D1_E2C || oper == pade::D2_E2E || oper == pade::INTERP_E2C);
+    const bool e_in = edge_input(op->kind, oper);
     const bool e_out = edge_output(op->kind, oper);
     const size_t plane = sizeof(double) * (size_t)na * nb;
     const size_t in_bytes = plane * (op->n + (e_in ? 1 : 0)), out_bytes = plane * (op->n + (e_out ? 1 : 0));
@@ -726,16 +741,18 @@ int pade_plan_export(int kind, int oper, int n, double dx, int periodic, int bc1
     if (int rc = operator_matrices(kind, oper, n, kind == pade::CF90 ? 1.0 : dx, periodic != 0, bc1, bcn,
                                    adjoint != 0, m))
         return fail(rc, "operator construction failed");
+    const int rows = m.n, cols = m.n_in ? m.n_in : m.n;   // a staggered cell <-> edge operator: n +- 1
     if (M == 0) {
-        M = n;
+        M = rows;
         for (int c : {32, 64, 16})
-            if (n % c == 0 && n / c <= pade::kMaxPartitions) { M = c; break; }
+            if (rows == cols && rows % c == 0 && rows / c <= pade::kMaxPartitions) { M = c; break; }
     }
+    if (M != rows && (rows != cols || rows % M)) return fail(400, "partition size must divide a square operator");
     std::vector<double> t;
     Variant v;
     if (int rc = build_variant(m, M, t, v)) return fail(rc == 1 ? 400 : rc, "factorisation failed");
-    int vals[32] = {n, v.M, v.P, periodic ? 1 : 0, v.nbl, v.nbr, pade::kOffBl, pade::kOffBr,
-                    pade::kOffCoef, v.coef_len, v.ncls, v.kq, v.len};
+    int vals[32] = {rows, v.M, v.P, periodic ? 1 : 0, v.nbl, v.nbr, pade::kOffBl, pade::kOffBr,
+                    pade::kOffCoef, v.coef_len, v.ncls, v.kq, v.len, cols};
     for (int k = 0; k < pade::kMaxPartitions; k++) vals[16 + k] = v.cls[k];
     std::memcpy(meta, vals, sizeof(vals));
     if (tab) {

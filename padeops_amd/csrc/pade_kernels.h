@@ -35,6 +35,9 @@ struct ApplyArgs {
     long long A1, B1, B2;  // pencil q -> base offset (q % A1)*B1 + (q / A1)*B2
     long long stride;      // element distance between consecutive points of a pencil
     int n, M, P, periodic, nbl, nbr;
+    int n_in;              // input points per pencil (the right boundary rows
+                           // read in[n_in - 1 - c]); n except for the
+                           // staggered cell <-> edge operators (generic path)
     int coef_len;          // length of the coefficient area
     int ncls;              // number of coefficient blocks
     int kq;                // reduced-system window: partitions p-kq..p+kq (2kq+1 >= P: all)

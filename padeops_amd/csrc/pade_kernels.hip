@@ -649,19 +649,20 @@ __global__ __launch_bounds__(256) void generic_apply(ApplyArgs<T> a, T *__restri
     const T *__restrict__ Q = lu + 10 * n;              // [4][4]
     T *y = ADJ ? work : a.out;
 
-    auto stencil_row = [&](const T *src, int i) -> T {
+    // B applied to `src` (src_n points per pencil) at row i
+    auto stencil_row = [&](const T *src, int src_n, int i) -> T {
         if (!a.periodic) {
             if (i < a.nbl) {
                 T s = T(0);
-                for (int c = 0; c < kBColsDev && c < n; c++)
+                for (int c = 0; c < kBColsDev && c < src_n; c++)
                     s += tab[kOffBl + i * kBColsDev + c] * src[base + (long long)c * st];
                 return s;
             }
             if (i >= n - a.nbr) {
                 const int r = n - 1 - i;
                 T s = T(0);
-                for (int c = 0; c < kBColsDev && c < n; c++)
-                    s += tab[kOffBr + r * kBColsDev + c] * src[base + (long long)(n - 1 - c) * st];
+                for (int c = 0; c < kBColsDev && c < src_n; c++)
+                    s += tab[kOffBr + r * kBColsDev + c] * src[base + (long long)(src_n - 1 - c) * st];
                 return s;
             }
         }
@@ -669,20 +670,18 @@ __global__ __launch_bounds__(256) void generic_apply(ApplyArgs<T> a, T *__restri
         for (int k = 0; k < 9; k++) {
             const T cf = tab[k];
             if (cf == T(0)) continue;
-            const int j = wrap_index(i - 4 + k, n, a.periodic);
+            const int j = wrap_index(i - 4 + k, src_n, a.periodic);
             s += cf * src[base + (long long)j * st];
         }
         return s;
     };
 
-    if (!ADJ) {
-        for (int i = 0; i < n; i++) y[base + (long long)i * st] = stencil_row(a.f, i);
-    } else {
-        for (int i = 0; i < n; i++) y[base + (long long)i * st] = a.f[base + (long long)i * st];
-    }
-    T zm1 = y[base], zm2 = T(0);
-    for (int i = 1; i < n; i++) {
-        T z = y[base + (long long)i * st] - lu[i * 6 + 0] * zm1 - lu[i * 6 + 1] * zm2;
+    // forward elimination; the forward operator forms each right-hand-side
+    // row as it goes (f and the output never overlap)
+    T zm1 = T(0), zm2 = T(0);
+    for (int i = 0; i < n; i++) {
+        const T r = ADJ ? a.f[base + (long long)i * st] : stencil_row(a.f, a.n_in, i);
+        const T z = i == 0 ? r : r - lu[i * 6 + 0] * zm1 - lu[i * 6 + 1] * zm2;
         y[base + (long long)i * st] = z;
         zm2 = zm1;
         zm1 = z;
@@ -705,7 +704,7 @@ __global__ __launch_bounds__(256) void generic_apply(ApplyArgs<T> a, T *__restri
         }
     }
     if (ADJ) {
-        for (int i = 0; i < n; i++) a.out[base + (long long)i * st] = stencil_row(work, i);
+        for (int i = 0; i < n; i++) a.out[base + (long long)i * st] = stencil_row(work, n, i);
     }
 }
 

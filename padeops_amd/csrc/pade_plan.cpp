@@ -13,6 +13,7 @@
 #include "pade_plan.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <utility>
 
@@ -98,6 +99,14 @@ Interior interior(int kind, int oper)
             const int o = oper == INTERP_E2C ? 0 : -1;          // E2C :538-545, C2E :585-597
             s.b[4 + o] = s.b[5 + o] = a; s.b[3 + o] = s.b[6 + o] = b;
             s.parity = 1; s.dxpow = 0;
+        } else if (oper == D1_C2C || oper == D1_E2E) {
+            // the collocated sixth-order first derivative on cells or edges
+            // (STAGG_CD06_files/D1RHS_C2C_common.F90:1, D1RHS_E2E_common.F90:1;
+            // alpha ComputeTri_allRoutines.F90:103,174)
+            al = 1.0 / 3.0;
+            const double a = (14.0 / 9.0) / 2.0, b = (1.0 / 9.0) / 4.0;
+            s.b[5] = a; s.b[3] = -a; s.b[6] = b; s.b[2] = -b;
+            s.parity = -1; s.dxpow = 1;
         } else {
             s.H = -1;
             return s;
@@ -221,6 +230,203 @@ void mirror_rows(const Interior &s, int bc, std::vector<Row> &A, std::vector<Row
     }
 }
 
+// ------------------------------------------------------------------------
+// Bounded staggered operators (cd06stagg.F90:197-232, init_nonperiodic).
+//
+// An even or odd end is the interior scheme applied to the mirror image of
+// the data about the wall: a cell array's wall lies half a cell beyond its
+// end point, an edge array's wall on its end point.  Input and output are
+// reflected with their own parities (a first derivative flips it).  This
+// reproduces every even / odd row of the reference's right-hand sides
+// (STAGG_CD06_files/*RHS*_common.F90) and tridiagonal systems
+// (ComputeTri_allRoutines.F90).  A one-sided end replaces the first rows by
+// the closure tables below (bottom end; the top end uses them reflected,
+// negated for a first derivative).  Order of assignment follows the
+// reference: right-hand side bottom rows, then top rows (the top wins where
+// they meet on very short grids); left-hand side top, then bottom.
+
+struct TriRow {
+    double dn, dg, up;
+};
+struct SidedClosure {
+    int nrows = 0;                       // 0: the operation has no one-sided form
+    std::vector<Row> B;                  // bottom rows, columns from the wall
+    TriRow A[3] = {};
+};
+
+SidedClosure stagg_sided(int oper)
+{
+    SidedClosure c;
+    const double w0s = 223.0 / 186.0, w1s = 61.0 / 62.0;            // cd06stagg.F90:56
+    if (oper == D1_E2C) {
+        // D1RHS_E2C_common.F90:2-5,12-16; ComputeTri_allRoutines.F90:6,26-29
+        const double al1 = 37.0 / 183.0, al0 = -1.0;
+        const double a0 = (al0 - 23.0) / 24.0, b0 = (7.0 - 9.0 * al0) / 8.0;
+        const double c0 = (9.0 * al0 + 1.0) / 8.0, d0 = -(al0 + 1.0) / 24.0;
+        const double a1 = 3.0 * (3.0 - 2.0 * al1) / 8.0, b1 = (22.0 * al1 - 1.0) / 8.0;
+        c.nrows = 2;
+        c.B = {{0, {{0, w0s * a0}, {1, w0s * b0}, {2, w0s * c0}, {3, w0s * d0}}},
+               {1, {{0, -w1s * b1 / 3.0}, {1, -w1s * a1}, {2, w1s * a1}, {3, w1s * b1 / 3.0}}}};
+        c.A[0] = {0.0, w0s, w0s * al0};
+        c.A[1] = {w1s * al1, w1s, w1s * al1};
+    } else if (oper == D1_C2E) {
+        // D1RHS_C2E_common.F90:2-4,12-14; ComputeTri_allRoutines.F90:71-74
+        const double a1 = 12.0 / 11.0;
+        c.nrows = 2;
+        c.B = {{0, {{0, -71.0 / 24.0}, {1, 47.0 / 8.0}, {2, -31.0 / 8.0}, {3, 23.0 / 24.0}}},
+               {1, {{0, -a1}, {1, a1}}}};
+        c.A[0] = {0.0, 1.0, 0.0};
+        c.A[1] = {1.0 / 22.0, 1.0, 1.0 / 22.0};
+    } else if (oper == D1_C2C) {
+        // Lele's closure, cd06stagg.F90:18-53; D1RHS_C2C_common.F90:9-27;
+        // ComputeTri_allRoutines.F90:135-146
+        const double p = 17.0 / 6.0, q = 1.5, r = 1.5, s = -1.0 / 6.0, ah = 1.0 / 3.0;
+        const double q_p = 0.75, al_p = 0.25;
+        const double al_pp = ((40 * ah - 1) * q + 7 * (4 * ah - 1) * s) / (16 * (ah + 2) * q + 8 * (1 - 4 * ah) * s);
+        const double q_pp = (al_pp + 2) / 3.0, r_pp = (4 * al_pp - 1) / 12.0;
+        const double w1 = (2 * ah + 1) / (2 * (q + s));
+        const double w2 = ((8 * ah + 7) * q - 6 * (2 * ah + 1) * r + (8 * ah + 7) * s) / (9 * (q + s));
+        const double w3 = (4 * (ah + 2) * q + 2 * (1 - 4 * ah) * s) / (9 * (q + s));
+        c.nrows = 3;
+        c.B = {{0, {{0, -w1 * p}, {1, w1 * q}, {2, w1 * r}, {3, w1 * s}}},
+               {1, {{0, -w2 * q_p}, {2, w2 * q_p}}},
+               {2, {{0, -w3 * r_pp}, {1, -w3 * q_pp}, {3, w3 * q_pp}, {4, w3 * r_pp}}}};
+        c.A[0] = {0.0, w1, 3.0 * w1};
+        c.A[1] = {w2 * al_p, w2, w2 * al_p};
+        c.A[2] = {w3 * al_pp, w3, w3 * al_pp};
+    } else if (oper == INTERP_C2E) {
+        // InterpRHS_C2E_common.F90:2-3,8-10; ComputeTri_allRoutines.F90:215,236-237
+        const double a1 = (9.0 + 10.0 / 6.0) / 8.0;
+        c.nrows = 2;
+        c.B = {{0, {{0, 15.0 / 8.0}, {1, -5.0 / 4.0}, {2, 3.0 / 8.0}}},
+               {1, {{0, a1 / 2.0}, {1, a1 / 2.0}}}};
+        c.A[0] = {0.0, 1.0, 0.0};
+        c.A[1] = {1.0 / 6.0, 1.0, 1.0 / 6.0};
+    } else if (oper == INTERP_E2C) {
+        // InterpRHS_E2C_common.F90:2-5,7; ComputeTri_allRoutines.F90:263,279
+        const double al0 = 1.0;
+        c.nrows = 1;
+        c.B = {{0, {{0, (5.0 - al0) / 16.0}, {1, (9.0 * al0 + 15.0) / 16.0},
+                    {2, (9.0 * al0 - 5.0) / 16.0}, {3, (1.0 - al0) / 16.0}}}};
+        c.A[0] = {0.0, 1.0, al0};
+    }
+    return c;
+}
+
+// bc: +-1 even / odd end, +-2 one-sided closure (parity by the sign)
+int build_stagg_bounded(int oper, int n, double dx, int bc1, int bcn, const Interior &s, OpMatrices &out)
+{
+    auto ok = [](int b) { return b == 1 || b == -1 || b == 2 || b == -2; };
+    if (!ok(bc1) || !ok(bcn)) return 324;
+    const bool ein = stagg_edge_in(oper), eout = stagg_edge_out(oper);
+    const int nin = n + (ein ? 1 : 0), m = n + (eout ? 1 : 0);
+    const int sb = bc1 > 0 ? 1 : -1, st = bcn > 0 ? 1 : -1;
+    const int ob = sb * s.parity, ot = st * s.parity;     // output parities
+    // mirror image of index j of an array of len points -> (index, sign)
+    auto mirror = [](int j, bool edge, int len, int lo, int hi, int &sg) {
+        sg = 1;
+        if (j < 0) { sg = lo; return edge ? -j : -1 - j; }
+        if (j >= len) { sg = hi; return edge ? 2 * (len - 1) - j : 2 * len - 1 - j; }
+        return j;
+    };
+    std::vector<double> B((size_t)m * nin, 0.0);
+    std::vector<TriRow> A(m);
+    auto mirror_row_B = [&](int i) {
+        double *r = &B[(size_t)i * nin];
+        for (int c = 0; c < nin; c++) r[c] = 0.0;
+        for (int k = 0; k < 9; k++) {
+            if (s.b[k] == 0.0) continue;
+            int sg;
+            const int j = mirror(i - 4 + k, ein, nin, sb, st, sg);
+            r[j] += sg * s.b[k];
+        }
+    };
+    for (int i = 0; i < m; i++) {
+        mirror_row_B(i);
+        double row[3] = {s.a[1], s.a[2], s.a[3]};
+        double full[3] = {0.0, 0.0, 0.0};     // coefficients of y[i-1], y[i], y[i+1]
+        for (int o = -1; o <= 1; o++) {
+            int sg;
+            const int j = mirror(i + o, eout, m, ob, ot, sg);
+            full[j - i + 1] += sg * row[o + 1];
+        }
+        A[i] = {full[0], full[1], full[2]};
+    }
+    const SidedClosure sc = stagg_sided(oper);
+    const double bsign = double(s.parity);
+    auto put_B = [&](const Row &r, bool top) {
+        const int i = top ? m - 1 - r.row : r.row;
+        double *d = &B[(size_t)i * nin];
+        for (int c = 0; c < nin; c++) d[c] = 0.0;
+        for (const Entry &e : r.e) d[top ? nin - 1 - e.col : e.col] += (top ? bsign : 1.0) * e.v;
+    };
+    const bool bot_sided = sc.nrows && std::abs(bc1) == 2, top_sided = sc.nrows && std::abs(bcn) == 2;
+    if (bot_sided)
+        for (const Row &r : sc.B) put_B(r, false);
+    if (top_sided) {
+        for (const Row &r : sc.B) put_B(r, true);
+    } else if (bot_sided) {
+        for (int q = 0; q < sc.nrows; q++) mirror_row_B(m - 1 - q);
+    }
+    if (top_sided)
+        for (int q = 0; q < sc.nrows; q++) A[m - 1 - q] = {sc.A[q].up, sc.A[q].dg, sc.A[q].dn};
+    if (bot_sided)
+        for (int q = 0; q < sc.nrows; q++) A[q] = sc.A[q];
+
+    out = OpMatrices();
+    out.n = m;
+    out.n_in = nin;
+    out.periodic = false;
+    out.H = s.H;
+    out.band.assign((size_t)m * 5, 0.0);
+    for (int i = 0; i < m; i++) {
+        if (i > 0) out.band[(size_t)i * 5 + 1] = A[i].dn;
+        out.band[(size_t)i * 5 + 2] = A[i].dg;
+        if (i + 1 < m) out.band[(size_t)i * 5 + 3] = A[i].up;
+    }
+    const double scale = std::pow(dx, -double(s.dxpow));
+    for (int k = 0; k < 9; k++) out.stencil[k] = s.b[k] * scale;
+    // rows that are not the plain stencil go to the explicit tables
+    auto plain = [&](int i) {
+        const double *r = &B[(size_t)i * nin];
+        for (int c = 0; c < nin; c++) {
+            const int k = c - i + 4;
+            const double want = (k >= 0 && k < 9) ? s.b[k] : 0.0;
+            if (r[c] != want) return false;
+        }
+        for (int k = 0; k < 9; k++) {
+            const int c = i - 4 + k;
+            if (s.b[k] != 0.0 && (c < 0 || c >= nin)) return false;
+        }
+        return true;
+    };
+    const int h = (m + 1) / 2;
+    int nbl = 0, nbr = 0;
+    for (int i = 0; i < h; i++)
+        if (!plain(i)) nbl = i + 1;
+    for (int i = m - 1; i >= h; i--)
+        if (!plain(i)) nbr = m - i;
+    if (nbl > kBRows || nbr > kBRows) return 21;
+    for (int q = 0; q < nbl; q++)
+        for (int c = 0; c < nin; c++) {
+            const double v = B[(size_t)q * nin + c];
+            if (v == 0.0) continue;
+            if (c >= kBCols) return 21;
+            out.bl[q][c] = v * scale;
+        }
+    for (int q = 0; q < nbr; q++)
+        for (int c = 0; c < nin; c++) {
+            const double v = B[(size_t)(m - 1 - q) * nin + c];
+            if (v == 0.0) continue;
+            const int cc = nin - 1 - c;
+            if (cc >= kBCols) return 21;
+            out.br[q][cc] = v * scale;
+        }
+    out.nbl = nbl;
+    out.nbr = nbr;
+    return 0;
+}
+
 }  // namespace
 
 int init_error(int kind, int n, bool periodic)
@@ -240,7 +446,10 @@ int build_operator(int kind, int oper, int n, double dx, bool periodic, int bc1,
     Interior s = interior(kind, oper);
     if (s.H < 0) return 21;
     if (kind == CD06 && oper == D2 && !periodic) return 21;  // composed from d1 by the caller
-    if (kind == CD06STAGG && !periodic) return 21;           // periodic staggered operators only
+    if (kind == CD06STAGG && !periodic) return build_stagg_bounded(oper, n, dx, bc1, bcn, s, out);
+    // the reference builds the collocated staggered-array derivatives for the
+    // bounded form only (cd06stagg.F90:224-227)
+    if (kind == CD06STAGG && (oper == D1_C2C || oper == D1_E2E)) return 21;
     if (kind != CD06 && !periodic) {
         auto ok = [](int b) { return b == 0 || b == 1 || b == -1; };
         if (!ok(bc1) || !ok(bcn)) return 324;
@@ -434,7 +643,11 @@ int invert(std::vector<double> &a, int m)
 int spike_factor(const OpMatrices &op, int M, SpikeFactors &out)
 {
     const int n = op.n;
-    if (M < 8 || n % M != 0) return 1;
+    // partitions of at least 8 rows; a bounded pencil solved whole (one
+    // partition, no reduced system) may be as short as 4 (the staggered
+    // operators on n = 5..7 cells)
+    const bool whole_bounded = !op.periodic && M == n && n >= 4;
+    if ((M < 8 && !whole_bounded) || n % M != 0) return 1;
     const int P = n / M;
     out = SpikeFactors();
     out.n = n;

@@ -18,10 +18,22 @@
 namespace pade {
 
 enum Kind { CD06 = 1, CD10 = 2, CF90 = 3, CD06STAGG = 4 };
-// cd06stagg (periodic): edge (E) and cell (C) staggered z operators; an E
-// array holds n + 1 planes, the last a copy of the first
+// cd06stagg: edge (E) and cell (C) staggered z operators; an E array holds
+// n + 1 planes (periodic: the last a copy of the first).  D1_C2C and D1_E2E
+// exist for the bounded form only, as in the reference.
 enum Oper { D1 = 1, D2 = 2, FILTER = 3,
-            D1_E2C = 11, D1_C2E = 12, D2_C2C = 13, D2_E2E = 14, INTERP_E2C = 15, INTERP_C2E = 16 };
+            D1_E2C = 11, D1_C2E = 12, D2_C2C = 13, D2_E2E = 14, INTERP_E2C = 15, INTERP_C2E = 16,
+            D1_C2C = 17, D1_E2E = 18 };
+
+// Input and output lengths of a staggered operation on n cells.
+inline bool stagg_edge_in(int oper)
+{
+    return oper == D1_E2C || oper == D2_E2E || oper == INTERP_E2C || oper == D1_E2E;
+}
+inline bool stagg_edge_out(int oper)
+{
+    return oper == D1_C2E || oper == D2_E2E || oper == INTERP_C2E || oper == D1_E2E;
+}
 
 constexpr int kBandW = 2;      // band half-width handled by the solver
 constexpr int kStencilH = 4;   // max stencil half-width (cf90)
@@ -32,7 +44,8 @@ constexpr int kSStride = 4;    // S columns: tail(p-1)[0..1], head(p+1)[0..1]
 
 // Dense description of one operator: A band rows and B = stencil + boundary rows.
 struct OpMatrices {
-    int n = 0;
+    int n = 0;                           // output points = rows of A
+    int n_in = 0;                        // input points (columns of B); 0: n
     bool periodic = true;
     // band[i][o+2] = A(i, i+o) (indices wrapped when periodic)
     std::vector<double> band;            // n * 5
@@ -40,7 +53,7 @@ struct OpMatrices {
     int H = 0;                           // stencil half-width actually used
     int nbl = 0, nbr = 0;                // special rows at the left / right end
     // left rows q: out[q] = sum_c bl[q][c] * in[c];
-    // right rows q: out[n-1-q] = sum_c br[q][c] * in[n-1-c]
+    // right rows q: out[n-1-q] = sum_c br[q][c] * in[n_in-1-c]
     double bl[kBRows][kBCols] = {{0}};
     double br[kBRows][kBCols] = {{0}};
 };
@@ -55,8 +68,10 @@ struct SpikeFactors {
 };
 
 // Build A and B of the forward operator.  For cd06 bounded, bc1/bcn are the
-// init-time codes (they only choose the LHS closure); otherwise the apply-time
-// codes.  Returns 0 or an error code (324 for a bad bc, 21 unsupported).
+// init-time codes (they only choose the LHS closure); for bounded cd06stagg
+// the init-time end codes (+-1: even / odd, +-2: one-sided closure, parity
+// by the sign); otherwise the apply-time codes.  Returns 0 or an error code
+// (324 for a bad bc, 21 unsupported).
 int build_operator(int kind, int oper, int n, double dx, bool periodic, int bc1, int bcn,
                    OpMatrices &out);
 

@@ -23,7 +23,7 @@ import numpy as np
 from . import _lib
 from ._lib import PADE_CD06, PADE_CD10, PADE_CF90, PADE_D1, PADE_D2, PADE_FILTER, PadeError
 from ._lib import (PADE_CD06STAGG, PADE_D1_E2C, PADE_D1_C2E, PADE_D2_C2C, PADE_D2_E2E, PADE_INTERP_E2C,
-                   PADE_INTERP_C2E)
+                   PADE_INTERP_C2E, PADE_D1_C2C, PADE_D1_E2E)
 
 try:  # torch is plumbing for device memory and streams
     import torch
@@ -320,20 +320,28 @@ class cf90(_CompactOp):
 
 
 class cd06stagg(_CompactOp):
-    """Sixth-order staggered compact z operators, periodic form
-    (src/derivatives/cd06stagg.F90; init_periodic :170-195).  Cell (C) arrays
-    are (n1, n2, n), edge (E) arrays (n1, n2, n + 1) with the last plane a copy
-    of the first; every operation acts along z.  The non-periodic form
-    (init_nonperiodic, :197) is not provided."""
+    """Sixth-order staggered compact z operators (src/derivatives/cd06stagg.F90).
+    Cell (C) arrays are (n1, n2, n), edge (E) arrays (n1, n2, n + 1); every
+    operation acts along z.  Periodic form (init_periodic :170-195): an E
+    array's last plane repeats its first.  Bounded form (init_nonperiodic
+    :197-232): each end is even or odd, optionally with the one-sided
+    closure.  Real (float64 / float32) and complex128 arrays."""
 
     _kind = PADE_CD06STAGG
 
-    def init(self, nx, dx, periodic=True):
-        """nx cells; raises PadeError(21) for nx <= 4 (GracefulExit 21,
-        cd06stagg.F90:182-184) or a non-periodic grid."""
-        if not periodic:
-            raise PadeError(21, "the non-periodic staggered operators are not provided")
-        rc = self._create(nx, dx, True, 0, 0)
+    def init(self, nx, dx, isTopEven=None, isBotEven=None, isTopSided=False, isBotSided=False):
+        """``init(nx, dx)``: periodic.  ``init(nx, dx, isTopEven, isBotEven
+        [, isTopSided, isBotSided])``: bounded ("Bot" is the k = 1 end, "Top"
+        the k = n end).  Raises PadeError(21) for nx <= 4 (GracefulExit 21,
+        cd06stagg.F90:182-184,215-217)."""
+        if (isTopEven is None) != (isBotEven is None):
+            raise ValueError("give both isTopEven and isBotEven (bounded) or neither (periodic)")
+        if isTopEven is None:
+            rc = self._create(nx, dx, True, 0, 0)
+        else:
+            bc1 = (1 if isBotEven else -1) * (2 if isBotSided else 1)
+            bcn = (1 if isTopEven else -1) * (2 if isTopSided else 1)
+            rc = self._create(nx, dx, False, bc1, bcn)
         if rc:
             raise PadeError(rc, "cd06stagg init")
         return 0
@@ -344,18 +352,27 @@ class cd06stagg(_CompactOp):
         n = self._n
         cin, cout = n1 * n2 * (n + e_in), n1 * n2 * (n + e_out)
         host = isinstance(f, np.ndarray)
+        cplx = _is_complex(f)
+        if cplx != _is_complex(df):
+            raise TypeError("f and df must both be real or both complex")
+        # a complex (n1, n2, m) array is a real (2 n1, n2, m) one
+        na = 2 * int(n1) if cplx else int(n1)
         if host:
             if f.size != cin or df.size != cout:
                 raise ValueError(f"array sizes must be n1*n2*{n + e_in} and n1*n2*{n + e_out}")
             lib = _lib.load()
-            fa = np.asfortranarray(f, dtype=np.float64)
-            if df.dtype != np.float64 or not (df.flags.f_contiguous or df.ndim == 1):
-                raise ValueError("df must be a Fortran-ordered float64 array")
+            want = np.complex128 if cplx else np.float64
+            fa = np.asfortranarray(f, dtype=want)
+            if df.dtype != want or not (df.flags.f_contiguous or df.ndim == 1):
+                raise ValueError(f"df must be a Fortran-ordered {np.dtype(want).name} array")
             dp = ctypes.POINTER(ctypes.c_double)
             _lib.check(lib.pade_op_apply_host(self._h, oper, 3, fa.ctypes.data_as(dp), df.ctypes.data_as(dp),
-                                              int(n1), int(n2), 0, 0))
+                                              na, int(n2), 0, 0))
             return df
-        if f.dtype != df.dtype or f.dtype not in (torch.float64, torch.float32):
+        if cplx:
+            if f.dtype != torch.complex128 or df.dtype != torch.complex128:
+                raise TypeError("complex arrays must be complex128")
+        elif f.dtype != df.dtype or f.dtype not in (torch.float64, torch.float32):
             raise TypeError("f and df must both be float64 or both float32")
         if not (f.is_cuda and df.is_cuda):
             raise TypeError("device arrays must live on the GPU")
@@ -366,9 +383,9 @@ class cd06stagg(_CompactOp):
         if f.data_ptr() == df.data_ptr():
             raise ValueError("f and df must not alias")
         lib = _lib.load()
-        fn = lib.pade_op_apply if f.dtype == torch.float64 else lib.pade_op_apply_f32
+        fn = lib.pade_op_apply_f32 if f.dtype == torch.float32 else lib.pade_op_apply
         _lib.check(fn(self._h, oper, 3, ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(df.data_ptr()),
-                      int(n1), int(n2), 0, 0, _stream_ptr()))
+                      na, int(n2), 0, 0, _stream_ptr()))
         return df
 
     def ddz_E2C(self, fE, dfC, n1, n2):
@@ -394,3 +411,11 @@ class cd06stagg(_CompactOp):
     def InterpZ_C2E(self, fC, IfE, n1, n2):
         """cd06stagg.F90:960-975."""
         return self._stagg(PADE_INTERP_C2E, fC, IfE, n1, n2, 0, 1)
+
+    def ddz_C2C(self, fC, dfC, n1, n2):
+        """cd06stagg.F90:883-892 (bounded form only)."""
+        return self._stagg(PADE_D1_C2C, fC, dfC, n1, n2, 0, 0)
+
+    def ddz_E2E(self, fE, dfE, n1, n2):
+        """cd06stagg.F90:905-914 (bounded form only)."""
+        return self._stagg(PADE_D1_E2E, fE, dfE, n1, n2, 1, 1)

@@ -134,6 +134,23 @@ program test_shim
     call put("stg_ie2c_im", aimag(gcc))
     call stg%destroy()
 
+    ! staggered z operators, bounded (init_nonperiodic, keyword form as in
+    ! src/incompressible/PadeDerOps.F90:100-117): top odd and one-sided,
+    ! bottom even; fe's last plane is now a point of its own
+    fe(:, :, n3 + 1) = 0.5_rk * f(:, :, 2)
+    call stg%init(n3, 1.0_rk / n3, isTopEven = .false., isBotEven = .true., &
+                  isTopSided = .true., isBotSided = .false.)
+    call stg%ddz_C2C(f, g, n1, n2)
+    call put("stgb_d1_c2c", g)
+    call stg%ddz_E2E(fe, ge, n1, n2)
+    call put("stgb_d1_e2e", ge)
+    fce(:, :, 1:n3) = cmplx(f, -f, kind=rk)
+    fce(:, :, n3 + 1) = cmplx(0.25_rk, 0.5_rk, kind=rk)
+    call stg%ddz_E2C(fce, gcc, n1, n2)
+    call put("stgb_e2c_re", real(gcc, kind=rk))
+    call put("stgb_e2c_im", aimag(gcc))
+    call stg%destroy()
+
     ! init error codes (the reference's ierr values)
     ierr = c10%init(4, 0.1_rk, .true., 0, 0)
     call put_int("ierr_cd10_n4", ierr)

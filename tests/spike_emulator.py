@@ -23,8 +23,8 @@ def export(kind, which, n, dx, periodic, bc1, bcn, adjoint, M=0):
     _lib.check(lib.pade_plan_export(KIND[kind], OPER[which], n, dx, int(periodic), bc1, bcn, int(adjoint), M,
                                     tab.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), tab.size, meta))
     keys = ["n", "M", "P", "periodic", "nbl", "nbr", "off_bl", "off_br", "off_coef", "coef_len", "ncls", "kq",
-            "len"]
-    m = dict(zip(keys, list(meta)[:13]))
+            "len", "n_in"]
+    m = dict(zip(keys, list(meta)[:14]))
     m["cls"] = list(meta)[16:32]
     return tab, m
 
@@ -99,3 +99,49 @@ def apply_pencil(kind, which, v, dx, periodic, bc1=0, bcn=0, adjoint=False, M=0)
         return _solve(tab, m, _stencil(tab, m, v, pad))
     u = _solve(tab, m, v)
     return _stencil(tab, m, u, pad)
+
+
+def stagg_rhs(tab, m, v):
+    """B of a bounded staggered operator (rows m["n"], input m["n_in"]) on v,
+    as the one-lane-per-pencil kernel forms it: explicit left / right rows,
+    the interior stencil in between (every tap must fall inside v)."""
+    n, nin = m["n"], m["n_in"]
+    assert v.size == nin
+    st = tab[:9]
+    bl = tab[m["off_bl"]:m["off_bl"] + 128].reshape(8, 16)
+    br = tab[m["off_br"]:m["off_br"] + 128].reshape(8, 16)
+    kk = min(16, nin)
+    out = np.zeros(n)
+    for r in range(n):
+        if r < m["nbl"]:
+            out[r] = bl[r, :kk] @ v[:kk]
+        elif r >= n - m["nbr"]:
+            out[r] = br[n - 1 - r, :kk] @ v[::-1][:kk]
+        else:
+            for k in range(9):
+                if st[k] != 0.0:
+                    j = r - 4 + k
+                    assert 0 <= j < nin, (r, k)
+                    out[r] += st[k] * v[j]
+    return out
+
+
+def apply_stagg(oper, v, dx, bc1, bcn, M=0):
+    """A bounded staggered operator (cd06stagg kind, oper code) through the
+    exported tables."""
+    lib = _lib.load()
+    meta = (ctypes.c_int * 32)()
+    _lib.check(lib.pade_plan_export(4, oper, 0 + _stagg_n(oper, v.size), dx, 0, bc1, bcn, 0, M, None, 0, meta))
+    tab = np.zeros(meta[12])
+    _lib.check(lib.pade_plan_export(4, oper, _stagg_n(oper, v.size), dx, 0, bc1, bcn, 0, M,
+                                    tab.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), tab.size, meta))
+    keys = ["n", "M", "P", "periodic", "nbl", "nbr", "off_bl", "off_br", "off_coef", "coef_len", "ncls", "kq",
+            "len", "n_in"]
+    m = dict(zip(keys, list(meta)[:14]))
+    m["cls"] = list(meta)[16:32]
+    return _solve(tab, m, stagg_rhs(tab, m, v)), m
+
+
+def _stagg_n(oper, nin):
+    # cells n from the input length: edge inputs (E2C, E2E d2, E2C interp, E2E d1) hold n + 1
+    return nin - 1 if oper in (11, 14, 15, 18) else nin

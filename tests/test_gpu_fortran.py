@@ -113,6 +113,26 @@ def test_fortran_cd06stagg(records):
         assert np.abs(got - expect).max() / np.abs(expect).max() < 1e-12, tag
 
 
+def test_fortran_cd06stagg_bounded(records):
+    """Bounded staggered operators through cd06staggstuff's init_nonperiodic
+    (keyword arguments), ddz_C2C, ddz_E2E and the complex ddz_E2C."""
+    f = records["input"]
+    n3 = f.shape[2]
+    dz = 1.0 / n3
+    bot, top = (True, False), (False, True)
+    fe = np.concatenate([f, 0.5 * f[:, :, 1:2]], axis=2)
+    ce = np.concatenate([f, np.full(f.shape[:2] + (1,), 0.25)], axis=2)
+    cei = np.concatenate([-f, np.full(f.shape[:2] + (1,), 0.5)], axis=2)
+    checks = [("stgb_d1_c2c", pd.stagg_bounded_apply("d1_c2c", f, dz, bot, top)),
+              ("stgb_d1_e2e", pd.stagg_bounded_apply("d1_e2e", fe, dz, bot, top)),
+              ("stgb_e2c_re", pd.stagg_bounded_apply("d1_e2c", ce, dz, bot, top)),
+              ("stgb_e2c_im", pd.stagg_bounded_apply("d1_e2c", cei, dz, bot, top))]
+    for tag, expect in checks:
+        got = records[tag]
+        assert got.shape == expect.shape, tag
+        assert np.abs(got - expect).max() / np.abs(expect).max() < 1e-12, tag
+
+
 def test_fortran_cd06der2_pencil(records):
     f = records["input"][:, 2, 4]
     n = f.shape[0]

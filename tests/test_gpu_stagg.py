@@ -69,8 +69,10 @@ def test_stagg_errors():
         s.init(4, 0.1)
     assert e.value.code == 21
     with pytest.raises(po.PadeError) as e:
-        s.init(16, 0.1, periodic=False)
+        s.init(4, 0.1, True, True)
     assert e.value.code == 21
+    with pytest.raises(ValueError):
+        s.init(16, 0.1, isTopEven=True)
     s.init(16, 0.1)
     f = torch.zeros((16, 4, 4), dtype=torch.float64, device=DEV)
     g = torch.zeros_like(f)

@@ -206,6 +206,36 @@ def main():
             table[f"{kind}_{which}_{'periodic' if periodic else 'bounded'}" + ("" if bc is None else f"_bc{bc[0]}{bc[1]}")] = row
             o.destroy()
         extra["fp64_gpts_by_operator"] = table
+        # staggered z operators on 512^2 pencils of 512 cells / 513 edges:
+        # Gpts/s counts cells, the HBM fraction the bytes of input and output
+        log("bench: staggered operators")
+        fE = po.fortran_empty((n, n, n + 1), device=dev)
+        gE = po.fortran_empty((n, n, n + 1), device=dev)
+        fE[:, :, :n].copy_(f)
+        fE[:, :, n].copy_(f[:, :, 0])
+        sops = [("ddz_E2C", True, False), ("ddz_C2E", False, True), ("d2dz2_C2C", False, False),
+                ("d2dz2_E2E", True, True), ("InterpZ_E2C", True, False), ("InterpZ_C2E", False, True),
+                ("ddz_C2C", False, False), ("ddz_E2E", True, True)]
+        stagg = {}
+        for form in ("periodic", "bounded_SE_SE"):
+            s = po.cd06stagg()
+            if form == "periodic":
+                s.init(n, dx)
+            else:
+                s.init(n, 1.0 / n, isTopEven=True, isBotEven=True, isTopSided=True, isBotSided=True)
+            row = {}
+            for name, ein, eout in sops:
+                if form == "periodic" and name in ("ddz_C2C", "ddz_E2E"):
+                    continue
+                src, dst = (fE if ein else f), (gE if eout else df)
+                t = timed(lambda: getattr(s, name)(src, dst, n, n), max(3, args.steps // 5), 2)
+                by = 8.0 * n * n * ((n + ein) + (n + eout))
+                row[name] = {"gpts": round(pts / (t * 1e-3) / 1e9, 1),
+                             "hbm_frac": round(by / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}
+            stagg[form] = row
+            s.destroy()
+        extra["stagg_z_fp64"] = stagg
+        del fE, gE
         # composite operator: divergence with the sums fused into the producing
         # kernels (64 B/pt) against three applies plus two array adds (96 B/pt)
         log("bench: divergence")

@@ -175,11 +175,15 @@ int pade_permute3(const void *src, void *dst, int elem_size, int n1, int n2, int
  * kernels consume for (kind, oper, n, dx, periodic, bc1, bcn, adjoint) with
  * partition size M (M = 0: library choice), and meta[32] =
  * {rows, M, P, periodic, nbl, nbr, off_bl, off_br, off_coef, coef_len, ncls,
- *  kq, len, cols, 0, 0, cls[0..15]}: rows = output points per pencil, cols =
- * input points (both n except for the staggered cell <-> edge operators,
- * which take M = rows); stencil at [0,9), left/right boundary rows (8 x 16,
- * the right ones reading input point cols - 1 - c) at off_bl/off_br, then at
- * off_coef ncls blocks of 10*M values
+ *  kq, len, cols, sys, tail, cls[0..15]}: rows = points solved per pencil,
+ * cols = input points (both n except for the bounded staggered operators
+ * with an edge array: M = 0 gives their one-lane-per-pencil form, rows =
+ * outputs; M > 0 their system form, sys = 1: right rows read input point
+ * rows - c, and with tail = 1 one more output row follows from right row 7
+ * and the table's slots 9 (sub-diagonal) and 10 (inverse diagonal));
+ * stencil at [0,9), left/right boundary rows (8 x 16, the right ones reading
+ * input point cols - 1 - c) at off_bl/off_br, then at off_coef ncls blocks
+ * of 10*M values
  * (M band-LU rows l1, l2, 1/pivot, u1/pivot, u2/pivot, 0 and M coupling rows
  * of 4) followed by the reduced-system rows (periodic: the 4 rows of 4P values
  * of partition 0, which partition p uses rotated by p; bounded: 4 rows for

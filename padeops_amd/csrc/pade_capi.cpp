@@ -37,6 +37,9 @@ int hip_fail(hipError_t e, const char *where)
 // device table (see pade_kernels.h).
 struct Variant {
     int M = 0, P = 0;
+    int n = 0, n_in = 0;   // rows solved and input points per pencil
+    bool rect = false;     // staggered system form (strided-row mode 2)
+    int tail = 0;          // mode 2: one output row after the n solved
     long long off = 0;  // element offset of this variant's table in the plan buffer
     int nbl = 0, nbr = 0;
     int coef_len = 0, ncls = 0, kq = 0;
@@ -69,6 +72,10 @@ int build_variant(const pade::OpMatrices &m, int M, std::vector<double> &tab, Va
     const int m4 = 4 * P;
     v.M = M;
     v.P = P;
+    v.n = m.n;
+    v.n_in = m.n_in ? m.n_in : m.n;
+    v.rect = m.rect;
+    v.tail = m.tail ? 1 : 0;
     v.nbl = m.nbl;
     v.nbr = m.nbr;
     // coefficient classes: partitions with bit-identical (LU, S) blocks share one
@@ -110,6 +117,8 @@ int build_variant(const pade::OpMatrices &m, int M, std::vector<double> &tab, Va
     tab.resize(tab.size() + v.len, 0.0);
     double *t = &tab[v.off];
     for (int k = 0; k < 9; k++) t[k] = m.stencil[k];
+    t[9] = m.tail_dn;      // system form of a staggered cell-to-edge operator
+    t[10] = m.tail_idg;
     for (int r = 0; r < pade::kBRowsDev; r++)
         for (int c = 0; c < pade::kBColsDev; c++) {
             t[pade::kOffBl + r * pade::kBColsDev + c] = m.bl[r][c];
@@ -216,11 +225,20 @@ int build_all_plans(pade_op_s *op)
                 // one more or one fewer than n); the partitioned kernels need a
                 // square B
                 const int m = pl.mats.n;
-                const bool square = pl.mats.n_in == 0 || pl.mats.n_in == m;
+                // bounded staggered operators with an edge array: the
+                // partitioned kernels run their system form (n rows)
+                const bool sys_form = op->kind == pade::CD06STAGG && !op->periodic &&
+                                      (pade::stagg_edge_in(oper) || pade::stagg_edge_out(oper));
+                pade::OpMatrices sys;
+                if (sys_form) {
+                    if (int rc = pade::build_stagg_system(oper, op->n, op->dx, cb1, cbn, sys))
+                        return fail(rc, "operator construction failed");
+                }
+                const pade::OpMatrices &pm = sys_form ? sys : pl.mats;
                 for (int M : {16, 32, 64}) {
-                    if (!square || m % M || m / M > pade::kMaxPartitions) continue;
+                    if (pm.n % M || pm.n / M > pade::kMaxPartitions) continue;
                     Variant v;
-                    if (build_variant(pl.mats, M, tab, v) == 0) pl.spike.push_back(v);
+                    if (build_variant(pm, M, tab, v) == 0) pl.spike.push_back(v);
                 }
                 if (m >= 4 && build_variant(pl.mats, m, tab, pl.generic) == 0) pl.has_generic = true;
                 op->plans[{oper, b1, bn, adj}] = std::move(pl);
@@ -358,6 +376,10 @@ int apply_one(pade_op_t op, const Plan &pl, int axis, const T *f, T *df, int na,
         v = &pl.generic;
     }
     a.tab = base + v->off;
+    a.n = v->n;
+    a.n_in = v->n_in;
+    a.tail = v->tail;
+    if (v->rect) mode = 2;   // strided rows (axis 3) of a staggered system form
     a.M = v->M;
     a.P = v->P;
     a.nbl = v->nbl;
@@ -741,18 +763,23 @@ int pade_plan_export(int kind, int oper, int n, double dx, int periodic, int bc1
     if (int rc = operator_matrices(kind, oper, n, kind == pade::CF90 ? 1.0 : dx, periodic != 0, bc1, bcn,
                                    adjoint != 0, m))
         return fail(rc, "operator construction failed");
+    // a bounded staggered operator with an edge array: M = 0 exports the
+    // one-lane-per-pencil form, M > 0 the partitioned kernels' system form
+    if (kind == pade::CD06STAGG && !periodic && M > 0 && (pade::stagg_edge_in(oper) || pade::stagg_edge_out(oper)))
+        if (int rc = pade::build_stagg_system(oper, n, dx, bc1, bcn, m)) return fail(rc, "operator construction failed");
     const int rows = m.n, cols = m.n_in ? m.n_in : m.n;   // a staggered cell <-> edge operator: n +- 1
     if (M == 0) {
         M = rows;
         for (int c : {32, 64, 16})
             if (rows == cols && rows % c == 0 && rows / c <= pade::kMaxPartitions) { M = c; break; }
     }
-    if (M != rows && (rows != cols || rows % M)) return fail(400, "partition size must divide a square operator");
+    if (M != rows && ((rows != cols && !m.rect) || rows % M))
+        return fail(400, "partition size must divide the rows of a square or system-form operator");
     std::vector<double> t;
     Variant v;
     if (int rc = build_variant(m, M, t, v)) return fail(rc == 1 ? 400 : rc, "factorisation failed");
     int vals[32] = {rows, v.M, v.P, periodic ? 1 : 0, v.nbl, v.nbr, pade::kOffBl, pade::kOffBr,
-                    pade::kOffCoef, v.coef_len, v.ncls, v.kq, v.len, cols};
+                    pade::kOffCoef, v.coef_len, v.ncls, v.kq, v.len, cols, m.rect ? 1 : 0, m.tail ? 1 : 0};
     for (int k = 0; k < pade::kMaxPartitions; k++) vals[16 + k] = v.cls[k];
     std::memcpy(meta, vals, sizeof(vals));
     if (tab) {

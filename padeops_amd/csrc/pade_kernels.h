@@ -35,9 +35,11 @@ struct ApplyArgs {
     long long A1, B1, B2;  // pencil q -> base offset (q % A1)*B1 + (q / A1)*B2
     long long stride;      // element distance between consecutive points of a pencil
     int n, M, P, periodic, nbl, nbr;
-    int n_in;              // input points per pencil (the right boundary rows
-                           // read in[n_in - 1 - c]); n except for the
-                           // staggered cell <-> edge operators (generic path)
+    int n_in;              // input points per pencil: n except for the
+                           // staggered cell <-> edge operators (generic path:
+                           // the right boundary rows read in[n_in - 1 - c];
+                           // strided-row mode 2: in[n - c], n_in = n or n + 1)
+    int tail;              // mode 2: one more output row after the n solved
     int coef_len;          // length of the coefficient area
     int ncls;              // number of coefficient blocks
     int kq;                // reduced-system window: partitions p-kq..p+kq (2kq+1 >= P: all)
@@ -48,6 +50,8 @@ struct ApplyArgs {
 
 size_t spike_lds_bytes(size_t tsize, int M, int P, int PW, bool adj, int mode, int coef_len, bool bnd);
 
+// mode 0: strided rows (y, z); 1: x pencils through LDS transposes; 2:
+// strided rows of a bounded staggered operator in system form (forward only)
 template <typename T>
 hipError_t launch_spike(const ApplyArgs<T> &a, int M, int PW, bool adj, int mode, hipStream_t s);
 

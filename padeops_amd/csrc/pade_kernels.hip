@@ -163,7 +163,10 @@ size_t spike_lds_bytes(size_t tsize, int M, int P, int PW, bool adj, int mode, i
     return (size_t)lds_elems(M, P, PW, adj, mode, coef_len, (int)tsize, bnd) * tsize;
 }
 
-template <typename T, int M>
+// Right rows read the window from its last point (RS = 0: column c is point
+// n - 1 - c) or from the first right-halo point (RS = 1: point n - c, the
+// input of a staggered edge-to-cell operator has one point more).
+template <typename T, int M, int RS = 0>
 __device__ __forceinline__ void boundary_rows(const T (&w)[M + 8], const T *rows, int nrows, bool right,
                                               T (&res)[kBRowsDev])
 {
@@ -172,7 +175,8 @@ __device__ __forceinline__ void boundary_rows(const T (&w)[M + 8], const T *rows
         T s = T(0);
         if (r < nrows) {
 #pragma unroll
-            for (int c = 0; c < kBColsDev; c++) s += rows[r * kBColsDev + c] * (right ? w[M + 3 - c] : w[4 + c]);
+            for (int c = 0; c < kBColsDev; c++)
+                s += rows[r * kBColsDev + c] * (right ? w[M + 3 + RS - c] : w[4 + c]);
         }
         res[r] = s;
     }
@@ -181,13 +185,13 @@ __device__ __forceinline__ void boundary_rows(const T (&w)[M + 8], const T *rows
 // B (or B^T) applied in place on the window: w[t+4] is point t on entry,
 // w[t] is row t on exit; special rows at the ends of a bounded pencil replace
 // the interior stencil there.
-template <typename T, int M>
+template <typename T, int M, int RS = 0>
 __device__ __forceinline__ void apply_stencil(T (&w)[M + 8], const T *__restrict__ tab, const T *bnd,
                                               bool edge_left, bool edge_right, int nbl, int nbr)
 {
     T lb[kBRowsDev], rb[kBRowsDev];
     if (edge_left) boundary_rows<T, M>(w, bnd, nbl, false, lb);
-    if (edge_right) boundary_rows<T, M>(w, bnd + 128, nbr, true, rb);
+    if (edge_right) boundary_rows<T, M, RS>(w, bnd + 128, nbr, true, rb);
     T cst[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) cst[k] = tab[k];
@@ -294,7 +298,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
 
     // --------------------------------------------------------------- 1. load
     T hv[kHaloChunk];                                    // x halo chunk (forward)
-    if constexpr (MODE == 0) {
+    if constexpr (MODE != 1) {
         // main rows: uniform base of this wave's first partition + per-lane offset
         const unsigned hoff = loff + (unsigned)((long long)half * M * stp * (long long)sizeof(T));
         if constexpr (!ADJ) {
@@ -306,7 +310,8 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const int jl = wrap_index(pM - 4 + k, n, !BND);
-                const int jr = wrap_index(pM + M + k, n, !BND);
+                // a staggered operator's input may hold one point more than its rows
+                const int jr = wrap_index(pM + M + k, MODE == 2 ? a.n_in : n, !BND);
                 w[k] = fq[(long long)jl * stp];
                 w[M + 4 + k] = fq[(long long)jr * stp];
             }
@@ -469,7 +474,18 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
 
     lds_barrier();   // coefficients visible
 
-    if constexpr (!ADJ) apply_stencil<T, M>(w, tab, bnd, edge_left, edge_right, a.nbl, a.nbr);
+    // staggered cell-to-edge operators: the system leaves out the last output
+    // row, which follows from its own right-hand-side row (the last row of
+    // the right-row table) and the solution next to it after the solve
+    T rtail = T(0);
+    if constexpr (MODE == 2) {
+        if (edge_right && a.tail) {
+#pragma unroll
+            for (int c = 0; c < kBColsDev; c++)
+                rtail += bnd[128 + (kBRowsDev - 1) * kBColsDev + c] * w[M + 4 - c];
+        }
+    }
+    if constexpr (!ADJ) apply_stencil<T, M, MODE == 2 ? 1 : 0>(w, tab, bnd, edge_left, edge_right, a.nbl, a.nbr);
 
     // ---------------------------------------------- 2. local band LU solve
 #pragma unroll
@@ -565,7 +581,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
     }
 
     // ---------------------------------------------------------------- store
-    if constexpr (MODE == 0) {
+    if constexpr (MODE != 1) {
         const unsigned hoff = loff + (unsigned)((long long)half * M * stp * (long long)sizeof(T));
         if (valid) {
             if constexpr (EPI) {
@@ -577,6 +593,16 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
             } else {
 #pragma unroll
                 for (int t = 0; t < M; t++) st_s(ob + (long long)(pw0 + t) * stp, hoff, w[t]);
+            }
+        }
+        if constexpr (MODE == 2) {
+            // the left-out last row (table slots 9, 10: its sub-diagonal entry
+            // and the inverse of its diagonal)
+            if (valid && edge_right && a.tail) {
+                T v = (rtail - tab[9] * w[M - 1]) * tab[10];
+                T *dst = ob + (long long)(pw0 + M) * stp;
+                if constexpr (EPI) v = a.alpha * v + a.beta * ld(dst, hoff);
+                st_s(dst, hoff, v);
             }
         }
     } else {
@@ -721,6 +747,17 @@ static hipError_t launch_spike_t(const ApplyArgs<T> &a, hipStream_t s)
     // the accumulate epilogue exists for the forward operator only
     if (ADJ && a.axpby) return hipErrorInvalidValue;
     const bool epi = !ADJ && a.axpby;
+    if constexpr (MODE == 2) {
+        // staggered operators: forward, bounded, strided rows only
+        if (ADJ || a.periodic || a.n_in < a.n || a.n_in > a.n + 1) return hipErrorInvalidValue;
+        if (epi)
+            hipLaunchKernelGGL((spike_apply<T, M, PW, false, 2, true, true>), dim3((unsigned)nblk),
+                               dim3(64 * waves), lds, s, a);
+        else
+            hipLaunchKernelGGL((spike_apply<T, M, PW, false, 2, true, false>), dim3((unsigned)nblk),
+                               dim3(64 * waves), lds, s, a);
+        return hipGetLastError();
+    }
     if (a.periodic) {
         if (epi)
             hipLaunchKernelGGL((spike_apply<T, M, PW, ADJ, MODE, false, !ADJ>), dim3((unsigned)nblk),
@@ -742,8 +779,11 @@ static hipError_t launch_spike_t(const ApplyArgs<T> &a, hipStream_t s)
 template <typename T, int M, int PW>
 static hipError_t launch_spike_mp(const ApplyArgs<T> &a, bool adj, int mode, hipStream_t s)
 {
-    if (!adj)
+    if (!adj) {
+        if (mode == 2) return launch_spike_t<T, M, PW, false, 2>(a, s);
         return mode == 0 ? launch_spike_t<T, M, PW, false, 0>(a, s) : launch_spike_t<T, M, PW, false, 1>(a, s);
+    }
+    if (mode == 2) return hipErrorInvalidValue;
     return mode == 0 ? launch_spike_t<T, M, PW, true, 0>(a, s) : launch_spike_t<T, M, PW, true, 1>(a, s);
 }
 

@@ -313,13 +313,18 @@ SidedClosure stagg_sided(int oper)
     return c;
 }
 
-// bc: +-1 even / odd end, +-2 one-sided closure (parity by the sign)
-int build_stagg_bounded(int oper, int n, double dx, int bc1, int bcn, const Interior &s, OpMatrices &out)
+// Dense B (m x nin, before 1/dx^p) and tridiagonal A (m rows) of a bounded
+// staggered operation.  bc: +-1 even / odd end, +-2 one-sided closure
+// (parity by the sign).
+int stagg_dense(int oper, int n, int bc1, int bcn, const Interior &s, std::vector<double> &B,
+                std::vector<TriRow> &A, int &m_out, int &nin_out)
 {
     auto ok = [](int b) { return b == 1 || b == -1 || b == 2 || b == -2; };
     if (!ok(bc1) || !ok(bcn)) return 324;
     const bool ein = stagg_edge_in(oper), eout = stagg_edge_out(oper);
     const int nin = n + (ein ? 1 : 0), m = n + (eout ? 1 : 0);
+    m_out = m;
+    nin_out = nin;
     const int sb = bc1 > 0 ? 1 : -1, st = bcn > 0 ? 1 : -1;
     const int ob = sb * s.parity, ot = st * s.parity;     // output parities
     // mirror image of index j of an array of len points -> (index, sign)
@@ -329,8 +334,8 @@ int build_stagg_bounded(int oper, int n, double dx, int bc1, int bcn, const Inte
         if (j >= len) { sg = hi; return edge ? 2 * (len - 1) - j : 2 * len - 1 - j; }
         return j;
     };
-    std::vector<double> B((size_t)m * nin, 0.0);
-    std::vector<TriRow> A(m);
+    B.assign((size_t)m * nin, 0.0);
+    A.assign(m, TriRow{0.0, 0.0, 0.0});
     auto mirror_row_B = [&](int i) {
         double *r = &B[(size_t)i * nin];
         for (int c = 0; c < nin; c++) r[c] = 0.0;
@@ -372,11 +377,20 @@ int build_stagg_bounded(int oper, int n, double dx, int bc1, int bcn, const Inte
         for (int q = 0; q < sc.nrows; q++) A[m - 1 - q] = {sc.A[q].up, sc.A[q].dg, sc.A[q].dn};
     if (bot_sided)
         for (int q = 0; q < sc.nrows; q++) A[q] = sc.A[q];
+    return 0;
+}
 
+// OpMatrices of the dense (B, A) restricted to rows [0, m): explicit rows for
+// every row that is not the plain stencil; right rows relative to input
+// point nin - 1 (generic form) or to point m (system form, `rect`).
+int stagg_tables(const std::vector<double> &B, const std::vector<TriRow> &A, int m, int nin, double dx,
+                 const Interior &s, bool rect, OpMatrices &out)
+{
     out = OpMatrices();
     out.n = m;
     out.n_in = nin;
     out.periodic = false;
+    out.rect = rect;
     out.H = s.H;
     out.band.assign((size_t)m * 5, 0.0);
     for (int i = 0; i < m; i++) {
@@ -414,12 +428,13 @@ int build_stagg_bounded(int oper, int n, double dx, int bc1, int bcn, const Inte
             if (c >= kBCols) return 21;
             out.bl[q][c] = v * scale;
         }
+    if (rect && nbr > kBRows - 1) return 21;     // the last table row holds the tail row
     for (int q = 0; q < nbr; q++)
         for (int c = 0; c < nin; c++) {
             const double v = B[(size_t)(m - 1 - q) * nin + c];
             if (v == 0.0) continue;
-            const int cc = nin - 1 - c;
-            if (cc >= kBCols) return 21;
+            const int cc = rect ? m - c : nin - 1 - c;
+            if (cc < 0 || cc >= kBCols) return 21;
             out.br[q][cc] = v * scale;
         }
     out.nbl = nbl;
@@ -427,7 +442,49 @@ int build_stagg_bounded(int oper, int n, double dx, int bc1, int bcn, const Inte
     return 0;
 }
 
+int build_stagg_bounded(int oper, int n, double dx, int bc1, int bcn, const Interior &s, OpMatrices &out)
+{
+    std::vector<double> B;
+    std::vector<TriRow> A;
+    int m = 0, nin = 0;
+    if (int rc = stagg_dense(oper, n, bc1, bcn, s, B, A, m, nin)) return rc;
+    return stagg_tables(B, A, m, nin, dx, s, false, out);
+}
+
 }  // namespace
+
+int build_stagg_system(int oper, int n, double dx, int bc1, int bcn, OpMatrices &out)
+{
+    const Interior s = interior(CD06STAGG, oper);
+    if (s.H < 0) return 21;
+    std::vector<double> B;
+    std::vector<TriRow> A;
+    int m = 0, nin = 0;
+    if (int rc = stagg_dense(oper, n, bc1, bcn, s, B, A, m, nin)) return rc;
+    if (!stagg_edge_out(oper)) return stagg_tables(B, A, m, nin, dx, s, true, out);
+    // n + 1 outputs: eliminate the last row L from row L - 1 (its only neighbour)
+    const int L = m - 1;
+    const double tdn = A[L].dn, tdg = A[L].dg;
+    const double f = A[L - 1].up / tdg;
+    A[L - 1].dg -= f * A[L].dn;
+    A[L - 1].up = 0.0;
+    std::vector<double> Bt(B.begin() + (size_t)L * nin, B.begin() + (size_t)m * nin);
+    for (int c = 0; c < nin; c++) B[(size_t)(L - 1) * nin + c] -= f * Bt[c];
+    B.resize((size_t)L * nin);
+    A.resize(L);
+    if (int rc = stagg_tables(B, A, L, nin, dx, s, true, out)) return rc;
+    const double scale = std::pow(dx, -double(s.dxpow));
+    for (int c = 0; c < nin; c++) {
+        if (Bt[c] == 0.0) continue;
+        const int cc = L - c;
+        if (cc < 0 || cc >= kBCols) return 21;
+        out.br[kBRows - 1][cc] = Bt[c] * scale;
+    }
+    out.tail = true;
+    out.tail_dn = tdn;
+    out.tail_idg = 1.0 / tdg;
+    return 0;
+}
 
 int init_error(int kind, int n, bool periodic)
 {

@@ -56,6 +56,12 @@ struct OpMatrices {
     // right rows q: out[n-1-q] = sum_c br[q][c] * in[n_in-1-c]
     double bl[kBRows][kBCols] = {{0}};
     double br[kBRows][kBCols] = {{0}};
+    // System form of a bounded staggered operator (strided-row kernels): the
+    // right rows read out[n-1-q] = sum_c br[q][c] * in[n-c] (in may hold
+    // n + 1 points), and with `tail` one more output row follows the n solved
+    // ones: out[n] = (sum_c br[kBRows-1][c] * in[n-c] - tail_dn * y[n-1]) * tail_idg.
+    bool rect = false, tail = false;
+    double tail_dn = 0.0, tail_idg = 0.0;
 };
 
 // Partitioned solve factors (all FP64 on the host).
@@ -74,6 +80,12 @@ struct SpikeFactors {
 // (324 for a bad bc, 21 unsupported).
 int build_operator(int kind, int oper, int n, double dx, bool periodic, int bc1, int bcn,
                    OpMatrices &out);
+
+// System form of a bounded staggered operation for the partitioned
+// strided-row kernels: the cell-to-edge operators (n + 1 outputs) leave their
+// last output row out of the solve (eliminated from the row above it) and
+// recover it afterwards; rows and columns as described with OpMatrices::rect.
+int build_stagg_system(int oper, int n, double dx, int bc1, int bcn, OpMatrices &out);
 
 // B^T and A^T of an operator (adjoint).
 void transpose_operator(const OpMatrices &fwd, OpMatrices &adj);

@@ -101,10 +101,17 @@ def apply_pencil(kind, which, v, dx, periodic, bc1=0, bcn=0, adjoint=False, M=0)
     return _stencil(tab, m, u, pad)
 
 
+def _sys_right(row, v, n):
+    """System form: column c of a right row multiplies input point n - c
+    (the kernel's window clamps indices past the input's end)."""
+    idx = np.clip(n - np.arange(16), 0, v.size - 1)
+    return row @ v[idx]
+
+
 def stagg_rhs(tab, m, v):
     """B of a bounded staggered operator (rows m["n"], input m["n_in"]) on v,
-    as the one-lane-per-pencil kernel forms it: explicit left / right rows,
-    the interior stencil in between (every tap must fall inside v)."""
+    as the kernels form it: explicit left / right rows, the interior stencil
+    in between (every tap must fall inside v); system form when m["sys"]."""
     n, nin = m["n"], m["n_in"]
     assert v.size == nin
     st = tab[:9]
@@ -116,7 +123,10 @@ def stagg_rhs(tab, m, v):
         if r < m["nbl"]:
             out[r] = bl[r, :kk] @ v[:kk]
         elif r >= n - m["nbr"]:
-            out[r] = br[n - 1 - r, :kk] @ v[::-1][:kk]
+            if m.get("sys"):
+                out[r] = _sys_right(br[n - 1 - r], v, n)
+            else:
+                out[r] = br[n - 1 - r, :kk] @ v[::-1][:kk]
         else:
             for k in range(9):
                 if st[k] != 0.0:
@@ -136,10 +146,16 @@ def apply_stagg(oper, v, dx, bc1, bcn, M=0):
     _lib.check(lib.pade_plan_export(4, oper, _stagg_n(oper, v.size), dx, 0, bc1, bcn, 0, M,
                                     tab.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), tab.size, meta))
     keys = ["n", "M", "P", "periodic", "nbl", "nbr", "off_bl", "off_br", "off_coef", "coef_len", "ncls", "kq",
-            "len", "n_in"]
-    m = dict(zip(keys, list(meta)[:14]))
+            "len", "n_in", "sys", "tail"]
+    m = dict(zip(keys, list(meta)[:16]))
     m["cls"] = list(meta)[16:32]
-    return _solve(tab, m, stagg_rhs(tab, m, v)), m
+    y = _solve(tab, m, stagg_rhs(tab, m, v))
+    if m["tail"]:
+        # the row left out of the solve (cell-to-edge operators' last output)
+        br = tab[m["off_br"]:m["off_br"] + 128].reshape(8, 16)
+        r = _sys_right(br[7], v, m["n"])
+        y = np.append(y, (r - tab[9] * y[-1]) * tab[10])
+    return y, m
 
 
 def _stagg_n(oper, nin):

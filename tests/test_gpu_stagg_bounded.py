@@ -87,6 +87,35 @@ def test_stagg_bounded_512(op, bot, top):
 
 
 @pytest.mark.parametrize("op", sorted(OPS))
+def test_stagg_bounded_512_many_pencils(op):
+    # enough pencils (256 x 128) for the library to pick 64-point partitions,
+    # as at 512^3; operators with an edge array run their system form
+    nz, n1, n2 = 512, 256, 128
+    dz = 1.0 / nz
+    bot, top = "SO", "SE"
+    s = _make(nz, dz, bot, top)
+    f = _field(n1, n2, _m(op, nz, pd.STAGG_EDGE_IN[op]), 9)
+    D = pd.stagg_bounded_operator(op, nz, dz, ENDS[bot], ENDS[top])
+    ref = np.asfortranarray((f.reshape(n1 * n2, -1, order="F") @ D.T).reshape(n1, n2, -1, order="F"))
+    ft = po.as_fortran(torch.tensor(f, device=DEV))
+    out = po.fortran_empty(ref.shape, device=DEV)
+    getattr(s, OPS[op])(ft, out, n1, n2)
+    torch.cuda.synchronize()
+    assert relerr(out.cpu().numpy(), ref) < 1e-12
+    # the accumulating form on the same launch shape: out = 2 op(f) - out
+    prev = out.clone()
+    import ctypes
+    lib = _lib.load()
+    code = {"d1_e2c": 11, "d1_c2e": 12, "d2_c2c": 13, "d2_e2e": 14, "interp_e2c": 15, "interp_c2e": 16,
+            "d1_c2c": 17, "d1_e2e": 18}[op]
+    _lib.check(lib.pade_op_apply_axpby(s.handle, code, 3, ctypes.c_double(2.0), ctypes.c_void_p(ft.data_ptr()),
+                                       ctypes.c_double(-1.0), ctypes.c_void_p(out.data_ptr()), n1, n2, 0, 0,
+                                       ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    torch.cuda.synchronize()
+    assert relerr(out.cpu().numpy(), 2 * ref - prev.cpu().numpy()) < 1e-12
+
+
+@pytest.mark.parametrize("op", sorted(OPS))
 def test_stagg_bounded_fp32_host_complex(op):
     nz, n1, n2 = 48, 10, 3
     dz = 1.0 / nz

@@ -88,8 +88,8 @@ def test_planner_stagg_bounded_matches_oracle(op):
     """Bounded staggered operators: the planner's mirror construction and
     one-sided tables, run through the exported tables the way the kernels run
     them, equal the oracle's statement-by-statement restatement of the
-    reference, for every end combination; square cell operators also through
-    the partitioned solve."""
+    reference, for every end combination; at n = 64 also through the
+    partitioned solve (the system form for operators with an edge array)."""
     rng = np.random.default_rng(3)
     for bc1 in STAGG_ENDS:
         for bcn in STAGG_ENDS:
@@ -98,10 +98,13 @@ def test_planner_stagg_bounded_matches_oracle(op):
                 v = rng.standard_normal(nin)
                 D = pd.stagg_bounded_operator(op, n, 0.3, STAGG_ENDS[bc1], STAGG_ENDS[bcn])
                 ref = D @ v
-                Ms = [0] + ([16, 32] if n == 64 and not pd.STAGG_EDGE_IN[op] and not pd.STAGG_EDGE_OUT[op] else [])
+                # partitioned forms: square cell operators as they are, the
+                # others in system form (n rows solved, plus the tail row)
+                Ms = [0] + ([16, 32, 64] if n == 64 else [])
                 for M in Ms:
                     out, m = se.apply_stagg(STAGG_CODES[op], v, 0.3, bc1, bcn, M)
-                    assert m["n"] == D.shape[0] and m["n_in"] == D.shape[1]
+                    assert m["n"] + m["tail"] == D.shape[0] and m["n_in"] == D.shape[1]
+                    assert m["sys"] == (M > 0 and (pd.STAGG_EDGE_IN[op] or pd.STAGG_EDGE_OUT[op]))
                     err = np.abs(out - ref).max() / np.abs(ref).max()
                     assert err < 1e-12, (op, bc1, bcn, n, M, err)
 

@@ -256,6 +256,17 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
     const bool edge_right = BND && p == P - 1;
 
     constexpr bool WHOLE = MODE == 1 && PW >= 4;
+    // x tile rows <-> (pencil, partition of the wave).  Partition-major rows
+    // (partition * NPEN + pencil: lane l owns row l) put the per-lane row
+    // accesses of the transposes on distinct LDS banks; pencil-major rows
+    // (pencil * PW + partition) put consecutive lanes on rows PW apart, an
+    // 8-way bank conflict at PW = 8.  Measured at 512^3: whole-pencil FP64
+    // x +7-8 %, pair-wave FP64 +1-2 %, pair-wave FP32 -2-3 % (it keeps
+    // pencil-major rows, whose load instructions span fewer pencils).
+    constexpr bool PMAJ = sizeof(T) == 8 || PW >= 4;
+    auto row_pen = [](int r) { return PMAJ ? r % NPEN : r / PW; };
+    auto row_part = [](int r) { return PMAJ ? r / NPEN : r % PW; };
+    auto tile_row = [](int pn, int pt) { return PMAJ ? pt * NPEN + pn : pn * PW + pt; };
     T *bnd = smem + (WHOLE ? 0 : (ADJ && MODE != 1 ? 12 : 4) * P * NPEN);   // [256] (bounded)
     T *coef = bnd + (BND ? 256 : 0);                    // [coef_len]
     constexpr int XT = xtile_points(sizeof(T), M), PAD = xtile_row(sizeof(T), M);
@@ -342,7 +353,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
 #pragma unroll
             for (int s = 0; s < XT / VL; s++) {
                 const int r = s * G + lane / LPR;            // tile row
-                const int rp = r / PW, rh = r % PW;
+                const int rp = row_pen(r), rh = row_part(r);
                 const int part = wave * PW + rh;
                 const bool ok = q0 + rp < a.npencils && part < P;
                 const int j = (ok ? part : 0) * M + c * XT + VL * (lane % LPR);
@@ -358,7 +369,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
 #pragma unroll
             for (int s = 0; s < HC; s++) {
                 const int r = s * GH + lane / HC;
-                const int rp = r / PW, rh = r % PW;
+                const int rp = row_pen(r), rh = row_part(r);
                 const int part = wave * PW + rh;
                 const int k = lane % HC;
                 bool ok = q0 + rp < a.npencils;
@@ -382,7 +393,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
         constexpr int G = 64 / LPR;
         constexpr int NCH = M / XT;
         constexpr int W0 = ADJ ? 0 : 4;
-        const int myrow = pen * PW + half;
+        const int myrow = tile_row(pen, half);
 #pragma unroll
         for (int c = 0; c < NCH; c++) {
 #pragma unroll
@@ -406,14 +417,14 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
                 if (c == 0) {
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
-                        const T t = xt[(pen * 2 + 1) * PAD + k];
+                        const T t = xt[tile_row(pen, 1) * PAD + k];
                         if (half == 0) w[M + 4 + k] = t;
                     }
                 }
                 if (c == NCH - 1) {
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
-                        const T t = xt[(pen * 2) * PAD + XT - 4 + k];
+                        const T t = xt[tile_row(pen, 0) * PAD + XT - 4 + k];
                         if (half == 1) w[k] = t;
                     }
                 }
@@ -425,12 +436,12 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
                 if (c == 0) {
                     const int hr = half + 1 < P ? half + 1 : 0;
 #pragma unroll
-                    for (int k = 0; k < 4; k++) w[M + 4 + k] = xt[(pen * PW + hr) * PAD + k];
+                    for (int k = 0; k < 4; k++) w[M + 4 + k] = xt[tile_row(pen, hr) * PAD + k];
                 }
                 if (c == NCH - 1) {
                     const int hl = half >= 1 ? half - 1 : P - 1;
 #pragma unroll
-                    for (int k = 0; k < 4; k++) w[k] = xt[(pen * PW + hl) * PAD + XT - 4 + k];
+                    for (int k = 0; k < 4; k++) w[k] = xt[tile_row(pen, hl) * PAD + XT - 4 + k];
                 }
             }
             wave_lds_sync();
@@ -447,7 +458,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const T own = xt[myrow * PAD + k];
-                    const T nxt = xt[(pen * 2 + 1) * PAD + k];
+                    const T nxt = xt[tile_row(pen, 1) * PAD + k];
                     if (half == 0) {
                         w[k] = own;
                         if (!pair) w[M + 4 + k] = nxt;
@@ -607,7 +618,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
         }
     } else {
         constexpr int G = 64 / LPR;
-        const int myrow = pen * PW + half;
+        const int myrow = tile_row(pen, half);
 #pragma unroll
         for (int c = 0; c < M / XT; c++) {
 #pragma unroll
@@ -631,7 +642,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
         const bool full_tile = q0 + NPEN <= a.npencils && (wave + 1) * PW <= P;
         auto store_row = [&](int c, int s) {
             const int r = s * G + lane / LPR;
-            const int rp = r / PW, rh = r % PW;
+            const int rp = row_pen(r), rh = row_part(r);
             const unsigned xo =
                 (unsigned)((rp * (long long)n + rh * M + VL * (lane % LPR)) * (long long)sizeof(T));
             vec_t v;
@@ -651,7 +662,7 @@ __global__ __launch_bounds__(512) void spike_apply(ApplyArgs<T> a)
 #pragma unroll
                 for (int s = 0; s < XT / VL; s++) {
                     const int r = s * G + lane / LPR;
-                    if (q0 + r / PW < a.npencils && wave * PW + r % PW < P) store_row(c, s);
+                    if (q0 + row_pen(r) < a.npencils && wave * PW + row_part(r) < P) store_row(c, s);
                 }
             }
         }

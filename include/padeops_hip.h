@@ -34,7 +34,11 @@
 extern "C" {
 #endif
 
-enum pade_kind { PADE_CD06 = 1, PADE_CD10 = 2, PADE_CF90 = 3, PADE_CD06STAGG = 4 };
+/* PADE_GAUSSIAN: the explicit Gaussian filter of width 4 dx
+ * (src/filters/gaussian.F90:11-17,137-185), periodic form, n = 1 or n >= 9;
+ * PADE_FILTER on any axis, dx ignored.  It is the compressible solver's
+ * filter of its artificial transport properties (cgrid.F90:338,1113). */
+enum pade_kind { PADE_CD06 = 1, PADE_CD10 = 2, PADE_CF90 = 3, PADE_CD06STAGG = 4, PADE_GAUSSIAN = 5 };
 /* PADE_CD06STAGG (n > 4) takes the staggered operations, along z (axis 3)
  * only, on edge (E) arrays of n + 1 planes and cell (C) arrays of n:
  *   PADE_D1_E2C      ddz_E2C      (cd06stagg.F90:820)

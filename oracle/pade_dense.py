@@ -25,6 +25,8 @@ Pinning (tests/test_oracle_*.py):
     with respect to reference output),
   * periodic staggered operators (``src/derivatives/cd06stagg.F90``) -> sixth
     order of accuracy on smooth periodic data at edges and cells,
+  * periodic Gaussian filter (``src/filters/gaussian.F90``) -> its transfer
+    function agf + 2 sum_m c_m cos(m k) on every Fourier mode,
   * complex z filter (``src/filters/cf90.F90:1230-1251``) -> composition of
     the pinned A (0, 0) and B (bc1, bcn); its periodic case is unpinned (the
     reference reads an unallocated factorization there).
@@ -173,7 +175,20 @@ def interior(kind, which):
             B[m] = c[k]
             B[-m] = c[k]
         return A, B
+    if kind == "gaussian" and which == "filter":
+        # explicit Gaussian filter, src/filters/gaussian.F90:11-17 (periodic
+        # stencil :137-185): A = I
+        g = gaussian_coeffs()
+        B = {0: g[0]}
+        for m in range(1, 5):
+            B[m] = B[-m] = g[m]
+        return {0: 1.0}, B
     raise ValueError(f"unknown operator {kind}/{which}")
+
+
+def gaussian_coeffs():
+    """agf..egf of ``src/filters/gaussian.F90:11-17`` (offsets 0..4)."""
+    return (3565.0 / 10368.0, 3091.0 / 12960.0, 1997.0 / 25920.0, 149.0 / 12960.0, 107.0 / 103680.0)
 
 
 def scale_power(kind, which):
@@ -357,6 +372,8 @@ def min_n(kind, periodic):
         return 6 if periodic else 8
     if kind == "cd10":
         return 8
+    if kind == "gaussian":
+        return 9
     return 10
 
 
@@ -364,7 +381,7 @@ def init_error(kind, n, periodic):
     """Error code ``init`` returns for size n (0 = ok)."""
     if n == 1 or n >= min_n(kind, periodic):
         return 0
-    return {"cd06": 3 if periodic else 2, "cd10": 2, "cf90": 7}[kind]
+    return {"cd06": 3 if periodic else 2, "cd10": 2, "cf90": 7, "gaussian": 21}[kind]
 
 
 def assemble(kind, which, n, dx=1.0, periodic=True, bc1=0, bcn=0):

@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libpadeops_hip.so")
 
-PADE_CD06, PADE_CD10, PADE_CF90, PADE_CD06STAGG = 1, 2, 3, 4
+PADE_CD06, PADE_CD10, PADE_CF90, PADE_CD06STAGG, PADE_GAUSSIAN = 1, 2, 3, 4, 5
 PADE_D1, PADE_D2, PADE_FILTER = 1, 2, 3
 PADE_D1_E2C, PADE_D1_C2E, PADE_D2_C2C, PADE_D2_E2E, PADE_INTERP_E2C, PADE_INTERP_C2E = 11, 12, 13, 14, 15, 16
 PADE_D1_C2C, PADE_D1_E2E = 17, 18

@@ -153,7 +153,7 @@ int operator_matrices(int kind, int oper, int n, double dx, bool periodic, int b
 
 bool oper_valid(int kind, int oper)
 {
-    if (kind == pade::CF90) return oper == pade::FILTER;
+    if (kind == pade::CF90 || kind == pade::GAUSSIAN) return oper == pade::FILTER;
     if (kind == pade::CD06STAGG) return oper >= pade::D1_E2C && oper <= pade::D1_E2E;
     return oper == pade::D1 || oper == pade::D2;
 }
@@ -197,7 +197,7 @@ int build_all_plans(pade_op_s *op)
 {
     std::vector<double> tab;
     std::vector<int> opers;
-    if (op->kind == pade::CF90) opers = {pade::FILTER};
+    if (op->kind == pade::CF90 || op->kind == pade::GAUSSIAN) opers = {pade::FILTER};
     else if (op->kind == pade::CD06STAGG) {
         opers = {pade::D1_E2C, pade::D1_C2E, pade::D2_C2C, pade::D2_E2E, pade::INTERP_E2C, pade::INTERP_C2E};
         if (!op->periodic) opers.insert(opers.end(), {pade::D1_C2C, pade::D1_E2E});
@@ -527,9 +527,12 @@ int pade_op_create(int kind, int n, double dx, int periodic, int bc1, int bcn, p
 {
     if (!out) return fail(400, "null output handle");
     *out = nullptr;
-    if (kind < pade::CD06 || kind > pade::CD06STAGG) return fail(400, "unknown operator kind");
+    if (kind < pade::CD06 || kind > pade::GAUSSIAN) return fail(400, "unknown operator kind");
     if (n < 1) return fail(400, "n must be positive");
-    if (kind != pade::CF90 && !(dx > 0.0) && n > 1) return fail(400, "dx must be positive");
+    const bool filter_kind = kind == pade::CF90 || kind == pade::GAUSSIAN;   // dx unused
+    if (!filter_kind && !(dx > 0.0) && n > 1) return fail(400, "dx must be positive");
+    if (kind == pade::GAUSSIAN && !periodic && n > 1)
+        return fail(21, "the Gaussian filter is provided in its periodic form");
     if (int ierr = pade::init_error(kind, n, periodic != 0))
         return fail(ierr, "grid too small for this operator");
     if (kind == pade::CD06STAGG && !periodic) {
@@ -541,7 +544,7 @@ int pade_op_create(int kind, int n, double dx, int periodic, int bc1, int bcn, p
     auto op = std::make_unique<pade_op_s>();
     op->kind = kind;
     op->n = n;
-    op->dx = kind == pade::CF90 ? 1.0 : dx;
+    op->dx = filter_kind ? 1.0 : dx;
     op->periodic = periodic != 0;
     op->init_bc1 = bc1;
     op->init_bcn = bcn;
@@ -755,12 +758,13 @@ int pade_plan_export(int kind, int oper, int n, double dx, int periodic, int bc1
                      int M, double *tab, long long tab_len, int *meta)
 {
     if (!meta) return fail(400, "null meta");
-    if (kind < pade::CD06 || kind > pade::CD06STAGG) return fail(400, "unknown operator kind");
+    if (kind < pade::CD06 || kind > pade::GAUSSIAN) return fail(400, "unknown operator kind");
     if (!oper_valid(kind, oper)) return fail(21, "operation not provided for this operator kind");
     if (int ierr = pade::init_error(kind, n, periodic != 0)) return fail(ierr, "grid too small");
     if (n < 4) return fail(21, "no table for n < 4");
     pade::OpMatrices m;
-    if (int rc = operator_matrices(kind, oper, n, kind == pade::CF90 ? 1.0 : dx, periodic != 0, bc1, bcn,
+    if (int rc = operator_matrices(kind, oper, n, kind == pade::CF90 || kind == pade::GAUSSIAN ? 1.0 : dx,
+                                   periodic != 0, bc1, bcn,
                                    adjoint != 0, m))
         return fail(rc, "operator construction failed");
     // a bounded staggered operator with an edge array: M = 0 exports the

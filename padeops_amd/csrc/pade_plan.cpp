@@ -76,6 +76,16 @@ Interior interior(int kind, int oper)
         s.b[1] = s.b[7] = 4.0e-5;
         s.b[0] = s.b[8] = -5.0e-6;
         s.H = 4; s.parity = 1; s.dxpow = 0;
+    } else if (kind == GAUSSIAN && oper == FILTER) {
+        // explicit Gaussian filter of width 4 dx, periodic form
+        // (src/filters/gaussian.F90:11-17 coefficients, :137-185 stencil): A = I
+        s.a[2] = 1.0;
+        s.b[4] = 3565.0 / 10368.0;
+        s.b[3] = s.b[5] = 3091.0 / 12960.0;
+        s.b[2] = s.b[6] = 1997.0 / 25920.0;
+        s.b[1] = s.b[7] = 149.0 / 12960.0;
+        s.b[0] = s.b[8] = 107.0 / 103680.0;
+        s.H = 4; s.parity = 1; s.dxpow = 0;
     } else if (kind == CD06STAGG) {
         // periodic staggered sixth-order scheme, cd06stagg.F90:170-195 (LHS
         // alphas) and the ComputeZ*_periodic RHS routines: output point i of a
@@ -492,6 +502,8 @@ int init_error(int kind, int n, bool periodic)
     if (kind == CD06) return periodic ? (n >= 6 ? 0 : 3) : (n >= 8 ? 0 : 2);
     if (kind == CD10) return n >= 8 ? 0 : 2;
     if (kind == CF90) return n >= 10 ? 0 : 7;
+    // the periodic Gaussian stencil spans 9 points (gaussian.F90:137-185)
+    if (kind == GAUSSIAN) return periodic && n >= 9 ? 0 : 21;
     // cd06stagg.F90:182-184: GracefulExit 21 for nx <= 4
     if (kind == CD06STAGG) return n > 4 ? 0 : 21;
     return 1;

@@ -17,7 +17,7 @@
 
 namespace pade {
 
-enum Kind { CD06 = 1, CD10 = 2, CF90 = 3, CD06STAGG = 4 };
+enum Kind { CD06 = 1, CD10 = 2, CF90 = 3, CD06STAGG = 4, GAUSSIAN = 5 };
 // cd06stagg: edge (E) and cell (C) staggered z operators; an E array holds
 // n + 1 planes (periodic: the last a copy of the first).  D1_C2C and D1_E2E
 // exist for the bounded form only, as in the reference.

@@ -14,13 +14,13 @@ MPI job) these classes raise :class:`PadeError` with the same code:
 11 / 12 cd10 / cd06 init failed, 21 cd06 second derivative, 4 / 5 / 6 metric
 or curvilinear terms requested, 1 invalid derivative method, 51 cf90 init
 failed, 52 invalid filter method.  The spectral ("four", "cheb") derivative
-methods and the gaussian / least-squares filters are outside this library and
-raise 13 / 14 / 52.
+methods, the least-squares filter and the bounded Gaussian filter are outside
+this library and raise 13 / 14 / 52; the periodic Gaussian filter is provided.
 """
 from __future__ import annotations
 
 from ._lib import PadeError
-from .operators import cd06, cd10, cf90
+from .operators import cd06, cd10, cf90, gaussian
 
 _AXES = ("x", "y", "z")
 
@@ -173,20 +173,22 @@ class filters:
         if len(rest) != 6:
             raise TypeError("init takes sizes, (periodicx, periodicy, periodicz) and (methodx, methody, methodz)")
         pers, methods = rest[0:3], rest[3:6]
-        for axis, m in zip(_AXES, methods):
-            if m in ("gaussian", "lstsq"):
-                raise PadeError(52, f"the {m} filter in {axis} is not provided by this compact-operator library")
-            if m != "cf90":
+        for axis, m, per in zip(_AXES, methods, pers):
+            if m == "lstsq" or (m == "gaussian" and not per):
+                raise PadeError(52, f"the {'bounded ' if m == 'gaussian' else ''}{m} filter in {axis} "
+                                    "is not provided by this compact-operator library")
+            if m not in ("cf90", "gaussian"):
                 raise PadeError(52, f"Incorrect method select in direction {axis.upper()}")
         if self.initialized:
             self.destroy()
         self.xsz, self.ysz, self.zsz = xsz, ysz, zsz
-        for axis, n, per in zip(_AXES, (nx, ny, nz), pers):
-            op = cf90()
+        for axis, n, per, m in zip(_AXES, (nx, ny, nz), pers, methods):
+            op = cf90() if m == "cf90" else gaussian()
             if op.init(n, bool(per)) != 0:          # filters.F90:82-184
-                raise PadeError(51, f"Initializing cf90 failed in {axis.upper()} ")
+                raise PadeError(51, f"Initializing {m} {'failed' if m == 'cf90' else 'filter failed'} in "
+                                    f"{axis.upper()} ")
             self._ops[axis] = op
-            self._method[axis] = "cf90"
+            self._method[axis] = m
         self.initialized = True
 
     def destroy(self):
@@ -229,5 +231,5 @@ class filters:
         return self._filter("z", f, ff, bc1, bcn)
 
     def operator(self, axis):
-        """The underlying cf90 operator of one axis ("x", "y" or "z")."""
+        """The underlying cf90 / gaussian operator of one axis ("x", "y" or "z")."""
         return self._ops[axis]

@@ -21,7 +21,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import PADE_CD06, PADE_CD10, PADE_CF90, PADE_D1, PADE_D2, PADE_FILTER, PadeError
+from ._lib import PADE_CD06, PADE_CD10, PADE_CF90, PADE_GAUSSIAN, PADE_D1, PADE_D2, PADE_FILTER, PadeError
 from ._lib import (PADE_CD06STAGG, PADE_D1_E2C, PADE_D1_C2E, PADE_D2_C2C, PADE_D2_E2E, PADE_INTERP_E2C,
                    PADE_INTERP_C2E, PADE_D1_C2C, PADE_D1_E2E)
 
@@ -32,9 +32,10 @@ except ImportError:  # pragma: no cover
 
 
 def fortran_empty(shape, dtype=None, device="cuda"):
-    """Uninitialised tensor of `shape` (n1, n2, n3) with Fortran strides."""
-    n1, n2, n3 = shape
-    return torch.empty((n3, n2, n1), dtype=dtype or torch.float64, device=device).permute(2, 1, 0)
+    """Uninitialised tensor of `shape` (n1, n2, n3[, ...]) with Fortran strides."""
+    shape = tuple(int(s) for s in shape)
+    nd = len(shape)
+    return torch.empty(shape[::-1], dtype=dtype or torch.float64, device=device).permute(*range(nd - 1, -1, -1))
 
 
 def as_fortran(t):
@@ -317,6 +318,29 @@ class cf90(_CompactOp):
         _lib.check(lib.pade_op_filter3_complex(self._h, ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(df.data_ptr()),
                                                int(na), int(nb), b1, bn, int(has_bc), _stream_ptr()))
         return df
+
+
+class gaussian(_CompactOp):
+    """Explicit Gaussian filter of width 4 dx, periodic form
+    (src/filters/gaussian.F90: coefficients :11-17, stencil :137-185); the
+    compressible solver's artificial-property filter.  The bounded form is
+    not provided."""
+
+    _kind = PADE_GAUSSIAN
+
+    def init(self, n, periodic):
+        """Returns 0, or 21 for a bounded grid or 1 < n < 9 (the periodic
+        stencil spans 9 points)."""
+        return self._create(n, 1.0, periodic, 0, 0)
+
+    def filter1(self, f, df, na, nb, bc1=None, bcn=None):
+        return self._apply(PADE_FILTER, 1, f, df, na, nb, bc1, bcn)
+
+    def filter2(self, f, df, na, nb, bc1=None, bcn=None):
+        return self._apply(PADE_FILTER, 2, f, df, na, nb, bc1, bcn)
+
+    def filter3(self, f, df, na, nb, bc1=None, bcn=None):
+        return self._apply(PADE_FILTER, 3, f, df, na, nb, bc1, bcn)
 
 
 class cd06stagg(_CompactOp):

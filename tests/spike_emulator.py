@@ -10,7 +10,7 @@ import numpy as np
 
 from padeops_amd import _lib
 
-KIND = {"cd06": 1, "cd10": 2, "cf90": 3}
+KIND = {"cd06": 1, "cd10": 2, "cf90": 3, "gaussian": 5}
 OPER = {"d1": 1, "d2": 2, "filter": 3}
 
 

@@ -117,3 +117,18 @@ def test_planner_stagg_bounded_codes():
         assert rc == 324
     # ddz_C2C / ddz_E2E have no periodic form
     assert lib.pade_plan_export(4, 17, 16, 0.1, 1, 0, 0, 0, 0, None, 0, meta) == 21
+
+
+@pytest.mark.parametrize("n,M", [(64, 16), (64, 32), (512, 64), (37, 37), (9, 9)])
+def test_planner_gaussian_periodic(n, M):
+    """The periodic Gaussian filter (identity A, 9-point B) through the
+    exported tables, forward and adjoint (its own transpose)."""
+    v = np.random.default_rng(n).standard_normal(n)
+    G = pd.dense_operator("gaussian", "filter", n, 1.0, True)
+    for adj in (False, True):
+        out = se.apply_pencil("gaussian", "filter", v, 1.0, True, 0, 0, adjoint=adj, M=M)
+        ref = (G.T if adj else G) @ v
+        assert np.abs(out - ref).max() / np.abs(ref).max() < 1e-13
+    meta = (ctypes.c_int * 32)()
+    lib = _lib.load()
+    assert lib.pade_plan_export(5, 3, 32, 1.0, 0, 0, 0, 0, 0, None, 0, meta) == 21   # bounded: not provided

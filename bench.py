@@ -236,6 +236,21 @@ def main():
             s.destroy()
         extra["stagg_z_fp64"] = stagg
         del fE, gE
+        # the compressible solver's RK45 step (Taylor-Green, 512^3, artificial
+        # properties and cf90 filter each substep) on GPU-resident fields
+        log("bench: compressible RK45 step")
+        from padeops_amd import cgrid as gcg
+        cgs = gcg.cgrid(n, n, n, dx, dx, dx, gam=5.0 / 3.0)
+        gcg.taylorgreen(cgs)
+        dt_cg = cgs.get_dt(0.5)
+        ms_cg = timed(lambda: cgs.advance_RK45(dt_cg), 2, 1)
+        extra["cgrid_rk45_fp64"] = {"grid": [n, n, n], "ms_per_step": round(ms_cg, 2),
+                                    "ms_per_substep": round(ms_cg / 5, 2),
+                                    "gpts_substeps": round(5 * pts / (ms_cg * 1e-3) / 1e9, 3),
+                                    "finite": bool(torch.isfinite(cgs.W).all())}
+        cgs.destroy()
+        del cgs
+        torch.cuda.empty_cache()
         # composite operator: divergence with the sums fused into the producing
         # kernels (64 B/pt) against three applies plus two array adds (96 B/pt)
         log("bench: divergence")

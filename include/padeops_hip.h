@@ -197,6 +197,34 @@ int pade_permute3(const void *src, void *dst, int elem_size, int n1, int n2, int
 int pade_plan_export(int kind, int oper, int n, double dx, int periodic, int bc1, int bcn,
                      int adjoint, int M, double *tab, long long tab_len, int *meta);
 
+/* Pointwise stages of the compressible solver's Runge-Kutta substep
+ * (src/compressible/cgrid.F90, one ideal-gas species), each one streaming
+ * pass on `stream` over FP64 arrays of n points; groups of fields are planes
+ * of one allocation (plane k at base + k * n): W the five conserved
+ * variables (rho, rho u, rho v, rho w, total energy), du the nine velocity
+ * gradients (du/dx, du/dy, du/dz, dv/dx, ..., dw/dz), grad_* three gradient
+ * components, d4 three fourth derivatives (x, y, z), fluxes the fifteen
+ * x / y / z fluxes of the five equations.  The derivatives and filters
+ * between them are the operator calls above.  Return 0 or 400 / 1000 + hip
+ * error. */
+int pade_cg_conserved(long long n, const double *rho, const double *u, const double *v, const double *w,
+                      const double *e, double *W, void *stream);                  /* cgrid.F90:799-811 */
+int pade_cg_primitive(long long n, const double *W, double gam, double Rgas, double *rho, double *u, double *v,
+                      double *w, double *e, double *p, double *T, void *stream);  /* :764-797, :813-820 */
+int pade_cg_strain(long long n, const double *du, double *func, double *dil, void *stream);  /* :1085-1087, :1117 */
+int pade_cg_lad_mu(long long n, double Cmu, const double *rho, double *mustar, void *stream);   /* :1110 */
+int pade_cg_lad_bulk(long long n, double Cbeta, double hx, double hy, double hz, const double *rho,
+                     const double *grad_rho, const double *d4, const double *du, const double *dil, double *bulk,
+                     void *stream);                                                  /* :1115-1158 */
+int pade_cg_lad_kap(long long n, double Ckap, double gam, double hx, double hy, double hz, const double *rho,
+                    const double *p, const double *T, const double *grad_e, const double *d4, double *kap,
+                    void *stream);                                                   /* :1163-1196 */
+int pade_cg_fluxes(long long n, const double *W, const double *u, const double *v, const double *w, const double *p,
+                   const double *du, const double *mu, const double *bulk, const double *kap, const double *grad_T,
+                   double *fluxes, void *stream);                       /* :898-1056, :1338-1444 */
+int pade_cg_rk_stage(long long n, double dt, double A, double B, const double *rhs, double *Q, double *W,
+                     void *stream);                                                  /* :697-699 */
+
 /* Message for the last non-zero return code of this thread. */
 const char *pade_last_error(void);
 

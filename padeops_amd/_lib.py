@@ -25,6 +25,8 @@ EXPORTS = [
     "pade_op_apply_host", "pade_op_filter3_complex", "pade_op_filter3_complex_host",
     "pade_op_dense_apply", "pade_op_dense_info", "pade_pencil_copy", "pade_permute3", "pade_plan_export",
     "pade_last_error", "pade_version",
+    "pade_cg_conserved", "pade_cg_primitive", "pade_cg_strain", "pade_cg_lad_mu", "pade_cg_lad_bulk",
+    "pade_cg_lad_kap", "pade_cg_fluxes", "pade_cg_rk_stage",
 ]
 
 _lib = None
@@ -71,6 +73,15 @@ def load():
     lib.pade_permute3.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p]
     lib.pade_plan_export.argtypes = [c_int, c_int, c_int, c_double, c_int, c_int, c_int, c_int, c_int, dp,
                                      c_ll, ip]
+    v, d = c_void_p, c_double
+    lib.pade_cg_conserved.argtypes = [c_ll, v, v, v, v, v, v, v]
+    lib.pade_cg_primitive.argtypes = [c_ll, v, d, d, v, v, v, v, v, v, v, v]
+    lib.pade_cg_strain.argtypes = [c_ll, v, v, v, v]
+    lib.pade_cg_lad_mu.argtypes = [c_ll, d, v, v, v]
+    lib.pade_cg_lad_bulk.argtypes = [c_ll, d, d, d, d, v, v, v, v, v, v, v]
+    lib.pade_cg_lad_kap.argtypes = [c_ll, d, d, d, d, d, v, v, v, v, v, v, v]
+    lib.pade_cg_fluxes.argtypes = [c_ll, v, v, v, v, v, v, v, v, v, v, v, v]
+    lib.pade_cg_rk_stage.argtypes = [c_ll, d, d, d, v, v, v, v]
     lib.pade_last_error.restype = ctypes.c_char_p
     lib.pade_last_error.argtypes = []
     lib.pade_version.argtypes = []

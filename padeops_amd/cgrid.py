@@ -4,13 +4,15 @@ src/compressible/cgrid.F90 (advance_RK45 :663-717, getRHS :822-1056, LAD
 species with the power-law viscosity model (IdealGasEOS.F90,
 PowerLawViscosity.F90, MixtureEOS.F90 ns = 1) on a periodic grid.
 
-Every derivative and filter runs through libpadeops_hip (the derivatives and
-filters facades: cd10 / cd06 first and second derivatives, cf90 and the
-periodic Gaussian filter); the flux divergences accumulate into the
-right-hand side inside the derivative kernels (rhs = rhs - d(flux)/dx in one
-pass).  Pointwise algebra is PyTorch on the same FP64 arrays.  On one GPU the
-reference's y / x / z pencil transposes are identities: every field is one
-Fortran-ordered (nx, ny, nz) array.
+Every stage runs in libpadeops_hip: the derivatives and filters (cd10 first
+and second derivatives, cf90 and the periodic Gaussian filter), and the
+pointwise algebra between them as fused streaming kernels (pade_cg_*: the
+conserved / primitive conversions, the LAD sensors and properties, stress,
+heat flux and all fifteen fluxes in one pass, the RK stage).  Sums of
+derivatives accumulate inside the derivative kernels (rhs = rhs - d(flux)/dx,
+mustar = mustar + h^6 d4 f).  On one GPU the reference's y / x / z pencil
+transposes are identities: every field is one Fortran-ordered (nx, ny, nz)
+array, and groups of fields are planes of one (nx, ny, nz, k) array.
 
 Conserved variables, as the reference orders them for one species:
 W[0] = rho, W[1:4] = rho (u, v, w), W[4] = total energy rho (e + |u|^2 / 2).
@@ -21,7 +23,10 @@ import math
 
 import torch
 
-from . import composite
+import ctypes
+
+from . import _lib
+from ._lib import PADE_D1, PADE_D2, PadeError
 from .facades import derivatives, filters
 from .operators import as_fortran, fortran_empty
 
@@ -39,6 +44,14 @@ def _fo(t):
     """t itself when it already has Fortran strides, else a Fortran copy (the
     library reads (nx, ny, nz) arrays in Fortran order)."""
     return t if t.permute(2, 1, 0).is_contiguous() else as_fortran(t)
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _s():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
 class cgrid:
@@ -70,6 +83,7 @@ class cgrid:
         self.p, self.T, self.e = e(), e(), e()
         self.mu, self.bulk, self.kap = e(), e(), e()
         self.W = fortran_empty((nx, ny, nz, 5), device=device)
+        self._t1, self._t2 = e(), e()        # work fields of the operator chains
         self.tsim = 0.0
         self.step = 0
         self.dt = 0.0
@@ -82,6 +96,13 @@ class cgrid:
         self.fil.destroy()
         self.gfil.destroy()
 
+    @property
+    def N(self):
+        return self.nx * self.ny * self.nz
+
+    def _planes(self, k):
+        return fortran_empty((self.nx, self.ny, self.nz, k), device=self.device)
+
     # ------------------------------------------------------------ EOS
     def get_e_from_p(self):
         """IdealGasEOS.F90:54-61."""
@@ -90,10 +111,6 @@ class cgrid:
     def get_T(self):
         """IdealGasEOS.F90:63-70."""
         torch.mul(self.e, (self.gam - 1.0) / self.Rgas, out=self.T)
-
-    def get_p(self):
-        """IdealGasEOS.F90:45-52."""
-        torch.mul(self.rho * self.e, self.gam - 1.0, out=self.p)
 
     def get_sos(self):
         """IdealGasEOS.F90:72-79."""
@@ -105,25 +122,14 @@ class cgrid:
         self.get_T()
 
     def get_primitive(self):
-        """cgrid.F90:764-797 (one species)."""
-        W = self.W
-        self.rho.copy_(W[:, :, :, 0])
-        onebyrho = 1.0 / self.rho
-        torch.mul(W[:, :, :, 1], onebyrho, out=self.u)
-        torch.mul(W[:, :, :, 2], onebyrho, out=self.v)
-        torch.mul(W[:, :, :, 3], onebyrho, out=self.w)
-        self.e.copy_(W[:, :, :, 4] * onebyrho - 0.5 * (self.u * self.u + self.v * self.v + self.w * self.w))
-        self.get_p()
-        self.get_T()
+        """cgrid.F90:764-797 followed by post_bc (:813-820), one kernel."""
+        _lib.check(_lib.load().pade_cg_primitive(self.N, _p(self.W), self.gam, self.Rgas, _p(self.rho), _p(self.u),
+                                                 _p(self.v), _p(self.w), _p(self.e), _p(self.p), _p(self.T), _s()))
 
     def get_conserved(self):
-        """cgrid.F90:799-811 (one species)."""
-        W = self.W
-        W[:, :, :, 0].copy_(self.rho)
-        W[:, :, :, 1].copy_(self.rho * self.u)
-        W[:, :, :, 2].copy_(self.rho * self.v)
-        W[:, :, :, 3].copy_(self.rho * self.w)
-        W[:, :, :, 4].copy_(self.rho * (self.e + 0.5 * (self.u * self.u + self.v * self.v + self.w * self.w)))
+        """cgrid.F90:799-811, one kernel."""
+        _lib.check(_lib.load().pade_cg_conserved(self.N, _p(self.rho), _p(self.u), _p(self.v), _p(self.w),
+                                                 _p(self.e), _p(self.W), _s()))
 
     def get_transport_properties(self):
         """MixtureEOS.F90:175-229 (ns = 1) with PowerLawViscosity.F90:38-75."""
@@ -138,70 +144,74 @@ class cgrid:
         torch.mul(self.mu, Cp / self.Pr, out=self.kap)
 
     # ------------------------------------------------------ operators
-    def gradient(self, f, dfdx, dfdy, dfdz):
-        """cgrid.F90:446-478."""
-        composite.gradient(None, self.der, f, dfdx, dfdy, dfdz, self.x_bc, self.y_bc, self.z_bc)
+    def _op(self, which, axis, f, out, alpha=None, beta=None):
+        """One derivative along ``axis`` (0, 1, 2) of a (nx, ny, nz) field,
+        optionally accumulated: out = alpha * d(f) + beta * out."""
+        name = "xyz"[axis]
+        if which == PADE_D2 and {"x": self.der.getMethodx, "y": self.der.getMethody,
+                                 "z": self.der.getMethodz}[name]() == "cd06":
+            raise PadeError(21, "CD06 is incomplete right now")      # derivatives.F90:524-525
+        sz = (self.nx, self.ny, self.nz)
+        na, nb = [s for i, s in enumerate(sz) if i != axis]
+        bc = (self.x_bc, self.y_bc, self.z_bc)[axis]
+        self.der.operator(name)._apply(which, axis + 1, f, out, na, nb, bc[0], bc[1], alpha=alpha, beta=beta)
 
-    def _d4(self, axis, f, t1, t2):
+    def gradient(self, f, g):
+        """cgrid.F90:446-478 into the three planes of g (y, x, z order)."""
+        for a in (1, 0, 2):
+            self._op(PADE_D1, a, f, g[:, :, :, a])
+
+    def _d4(self, f, axis, t, out, scale=None):
         """Fourth derivative along one axis as the second derivative applied
-        twice (cgrid.F90:1089-1105): result in t2."""
-        d2 = {"x": self.der.d2dx2, "y": self.der.d2dy2, "z": self.der.d2dz2}[axis]
-        d2(f, t1)
-        d2(t1, t2)
-        return t2
+        twice (cgrid.F90:1089-1105); with ``scale``, out += scale * d4 f."""
+        self._op(PADE_D2, axis, f, t)
+        if scale is None:
+            self._op(PADE_D2, axis, t, out)
+        else:
+            self._op(PADE_D2, axis, t, out, alpha=scale, beta=1.0)
 
     def filter(self, arr, fil, numtimes=1):
         """cgrid.F90:1259-1336: y, then x, then z, each ``numtimes`` times;
-        the result replaces ``arr``."""
-        t1, t2 = self._empty(), self._empty()
-        for f1 in (fil.filtery, fil.filterx, fil.filterz):
-            f1(arr, t1)
-            for _ in range(numtimes - 1):
-                f1(t1, t2)
-                t1, t2 = t2, t1
-            arr.copy_(t1)
+        the result replaces ``arr`` (buffers alternate, no copies)."""
+        seq = [f1 for f1 in (fil.filtery, fil.filterx, fil.filterz) for _ in range(numtimes)]
+        bufs = (self._t1, self._t2)
+        src = arr
+        for k, f1 in enumerate(seq):
+            dst = arr if k == len(seq) - 1 else bufs[k % 2]
+            f1(src, dst)
+            src = dst
 
     # ------------------------------------------------------------- LAD
     def getLAD(self, du):
         """cgrid.F90:1058-1257 (no species diffusivity for one species)."""
-        dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz = du
-        t1, t2 = self._empty(), self._empty()
-        dx6, dy6, dz6 = self.dx ** 6, self.dy ** 6, self.dz ** 6
-        # artificial shear viscosity (:1083-1113)
-        func = _fo(torch.sqrt(dudx ** 2 + 0.5 * (dvdx + dudy) ** 2 + 0.5 * (dwdx + dudz) ** 2
-                              + dvdy ** 2 + 0.5 * (dwdy + dvdz) ** 2 + dwdz ** 2))
-        mustar = self._d4("x", func, t1, t2) * dx6
-        mustar += self._d4("z", func, t1, t2) * dz6
-        mustar += self._d4("y", func, t1, t2) * dy6
-        mustar = _fo(self.Cmu * self.rho * torch.abs(mustar))
+        lib, N, st = _lib.load(), self.N, _s()
+        hx, hy, hz = self.dx, self.dy, self.dz
+        t = self._t1
+        func, dil = self._empty(), self._empty()
+        _lib.check(lib.pade_cg_strain(N, _p(du), _p(func), _p(dil), st))
+        # artificial shear viscosity (:1083-1113): x, z, y as the reference sums
+        mustar = self._empty()
+        mustar.zero_()
+        for a, h in ((0, hx), (2, hz), (1, hy)):
+            self._d4(func, a, t, mustar, scale=h ** 6)
+        _lib.check(lib.pade_cg_lad_mu(N, self.Cmu, _p(self.rho), _p(mustar), st))
         self.filter(mustar, self.gfil, 2)
         # artificial bulk viscosity (:1115-1161)
-        func = _fo(dudx + dvdy + dwdz)
-        g1, g2, g3 = self._empty(), self._empty(), self._empty()
-        self.gradient(self.rho, g1, g2, g3)
-        g1.mul_(g1)
-        g2.mul_(g2)
-        g3.mul_(g3)
-        den = g1 + g2 + g3 + _EPS
-        bulkstar = self._d4("x", func, t1, t2) * self.dx ** 4 * (self.dx * g1 / den) ** 2
-        bulkstar += self._d4("z", func, t1, t2) * self.dz ** 4 * (self.dz * g3 / den) ** 2
-        bulkstar += self._d4("y", func, t1, t2) * self.dy ** 4 * (self.dy * g2 / den) ** 2
-        curl2 = (dwdy - dvdz) ** 2 + (dudz - dwdx) ** 2 + (dvdx - dudy) ** 2
-        dil2 = func * func
-        fsw = dil2 / (dil2 + curl2 + _EPS)
-        fsw = torch.where(func >= 0.0, torch.zeros_like(fsw), fsw)
-        bulkstar = _fo(self.Cbeta * self.rho * fsw * torch.abs(bulkstar))
+        g, d4 = self._planes(3), self._planes(3)
+        self.gradient(self.rho, g)
+        for a in (0, 2, 1):
+            self._d4(dil, a, t, d4[:, :, :, a])
+        bulkstar = func          # the sensor is no longer needed
+        _lib.check(lib.pade_cg_lad_bulk(N, self.Cbeta, hx, hy, hz, _p(self.rho), _p(g), _p(d4), _p(du), _p(dil),
+                                        _p(bulkstar), st))
         self.filter(bulkstar, self.gfil, 2)
         # artificial conductivity (:1163-1199)
-        self.gradient(self.e, g1, g2, g3)
-        g1.mul_(g1)
-        g2.mul_(g2)
-        g3.mul_(g3)
-        den = g1 + g2 + g3 + _EPS
-        kapstar = self._d4("x", self.e, t1, t2) * self.dx ** 4 * (self.dx * g1 / den)
-        kapstar += self._d4("z", self.e, t1, t2) * self.dz ** 4 * (self.dz * g3 / den)
-        kapstar += self._d4("y", self.e, t1, t2) * self.dy ** 4 * (self.dy * g2 / den)
-        kapstar = _fo(self.Ckap * self.rho * self.get_sos() * torch.abs(kapstar) / self.T)
+        self.gradient(self.e, g)
+        for a in (0, 2, 1):
+            self._d4(self.e, a, t, d4[:, :, :, a])
+        kapstar = dil
+        _lib.check(lib.pade_cg_lad_kap(N, self.Ckap, self.gam, hx, hy, hz, _p(self.rho), _p(self.p), _p(self.T),
+                                       _p(g), _p(d4), _p(kapstar), st))
         self.filter(kapstar, self.gfil, 2)
         # (:1252-1254)
         self.mu += mustar
@@ -212,59 +222,25 @@ class cgrid:
     def getRHS(self, rhs):
         """cgrid.F90:822-896 with getRHS_x/y/z (:898-1056), get_tau
         (:1338-1390) and get_q (:1392-1444).  ``rhs`` is (nx, ny, nz, 5)."""
-        du = [self._empty() for _ in range(9)]
-        dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz = du
-        self.gradient(self.u, dudx, dudy, dudz)
-        self.gradient(self.v, dvdx, dvdy, dvdz)
-        self.gradient(self.w, dwdx, dwdy, dwdz)
+        lib, N, st = _lib.load(), self.N, _s()
+        du = self._planes(9)
+        self.gradient(self.u, du[:, :, :, 0:3])
+        self.gradient(self.v, du[:, :, :, 3:6])
+        self.gradient(self.w, du[:, :, :, 6:9])
         self.get_transport_properties()
         self.getLAD(du)
-        # stress tensor (get_tau)
-        mu, bulk = self.mu, self.bulk
-        bambda = (4.0 / 3.0) * mu + bulk
-        lam = bulk - (2.0 / 3.0) * mu
-        tauxy = mu * (dudy + dvdx)
-        tauxz = mu * (dudz + dwdx)
-        tauyz = mu * (dvdz + dwdy)
-        tauxx = bambda * dudx + lam * (dvdy + dwdz)
-        tauyy = bambda * dvdy + lam * (dudx + dwdz)
-        tauzz = bambda * dwdz + lam * (dudx + dvdy)
-        del du, dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz
-        # heat flux (get_q): q = -kap grad T
-        qx, qy, qz = self._empty(), self._empty(), self._empty()
-        self.gradient(self.T, qx, qy, qz)
-        qx.mul_(-self.kap)
-        qy.mul_(-self.kap)
-        qz.mul_(-self.kap)
+        gT = self._planes(3)
+        self.gradient(self.T, gT)
+        fl = self._planes(15)
+        _lib.check(lib.pade_cg_fluxes(N, _p(self.W), _p(self.u), _p(self.v), _p(self.w), _p(self.p), _p(du),
+                                      _p(self.mu), _p(self.bulk), _p(self.kap), _p(gT), _p(fl), st))
+        del du, gT
+        # rhs = 0; rhs -= d(flux)/dx, then y, then z (getRHS_x, _y, _z), each
+        # difference accumulated inside the derivative kernel
         rhs.zero_()
-        W, u, v, w, p = self.W, self.u, self.v, self.w, self.p
-        rhou, rhov, rhow, TE = W[:, :, :, 1], W[:, :, :, 2], W[:, :, :, 3], W[:, :, :, 4]
-        flux = self._empty()
-
-        def acc(axis, k, val):
-            # rhs[k] -= d(flux)/d(axis), accumulated inside the derivative kernel
-            flux.copy_(val)
-            bc = {"x": self.x_bc, "y": self.y_bc, "z": self.z_bc}[axis]
-            composite._d1(self.der, axis, flux, rhs[:, :, :, k], bc, alpha=-1.0, beta=1.0)
-
-        # x fluxes (getRHS_x)
-        acc("x", 0, rhou)
-        acc("x", 1, rhou * u + p - tauxx)
-        acc("x", 2, rhou * v - tauxy)
-        acc("x", 3, rhou * w - tauxz)
-        acc("x", 4, (TE + p - tauxx) * u - v * tauxy - w * tauxz + qx)
-        # y fluxes (getRHS_y)
-        acc("y", 0, rhov)
-        acc("y", 1, rhov * u - tauxy)
-        acc("y", 2, rhov * v + p - tauyy)
-        acc("y", 3, rhov * w - tauyz)
-        acc("y", 4, (TE + p - tauyy) * v - u * tauxy - w * tauyz + qy)
-        # z fluxes (getRHS_z)
-        acc("z", 0, rhow)
-        acc("z", 1, rhow * u - tauxz)
-        acc("z", 2, rhow * v - tauyz)
-        acc("z", 3, rhow * w + p - tauzz)
-        acc("z", 4, (TE + p - tauzz) * w - u * tauxz - v * tauyz + qz)
+        for a in (0, 1, 2):
+            for k in range(5):
+                self._op(PADE_D1, a, fl[:, :, :, 5 * a + k], rhs[:, :, :, k], alpha=-1.0, beta=1.0)
 
     # ------------------------------------------------------------ step
     def advance_RK45(self, dt=None):
@@ -275,26 +251,23 @@ class cgrid:
         if self.step == 0:
             self.get_e_from_p()
             self.get_T()
-        rhs = fortran_empty((self.nx, self.ny, self.nz, 5), device=self.device)
-        Qtmp = torch.zeros_like(rhs)
+        lib, N = _lib.load(), self.N
+        rhs = self._planes(5)
+        Qtmp = self._planes(5)
+        Qtmp.zero_()
         Qtmpt = 0.0
         for isub in range(5):
             self.get_conserved()
             if not bool(torch.isfinite(self.W).all()):
                 raise FloatingPointError(f"NaN encountered in solution at substep {isub + 1} of step {self.step + 1}")
             self.getRHS(rhs)
-            Qtmp.mul_(RK45_A[isub]).add_(rhs, alpha=self.dt)
+            _lib.check(lib.pade_cg_rk_stage(N, self.dt, RK45_A[isub], RK45_B[isub], _p(rhs), _p(Qtmp), _p(self.W),
+                                            _s()))
             Qtmpt = self.dt + RK45_A[isub] * Qtmpt
-            self.W.add_(Qtmp, alpha=RK45_B[isub])
             self.tsim += RK45_B[isub] * Qtmpt
             for k in range(5):
-                arr = self.W[:, :, :, k]
-                tmp = self._empty()
-                tmp.copy_(arr)
-                self.filter(tmp, self.fil, 1)
-                arr.copy_(tmp)
+                self.filter(self.W[:, :, :, k], self.fil, 1)
             self.get_primitive()
-            self.post_bc()
         self.step += 1
 
     def get_dt(self, CFL):

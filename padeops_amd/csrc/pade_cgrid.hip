@@ -1,0 +1,279 @@
+// Pointwise stages of the compressible solver's Runge-Kutta substep
+// (src/compressible/cgrid.F90, one ideal-gas species): each kernel is one
+// streaming pass over the fields that a group of the reference's array
+// statements reads and writes, so that a substep moves each field a few
+// times instead of once per statement.  Fields are FP64 arrays of N points;
+// groups of fields (the five conserved variables, the nine velocity
+// gradients, three gradient components, fifteen fluxes) are planes of one
+// allocation, plane k at base + k * N.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "padeops_hip.h"
+
+namespace pade {
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(long long n)
+{
+    const long long b = (n + kBlock - 1) / kBlock;
+    return (unsigned)std::min<long long>(std::max<long long>(b, 1), 1LL << 20);
+}
+
+#define GRID_STRIDE(i, n) \
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long long)gridDim.x * blockDim.x)
+
+// get_conserved, cgrid.F90:799-811
+__global__ __launch_bounds__(kBlock) void k_conserved(long long n, const double *__restrict__ rho,
+                                                      const double *__restrict__ u, const double *__restrict__ v,
+                                                      const double *__restrict__ w, const double *__restrict__ e,
+                                                      double *__restrict__ W)
+{
+    GRID_STRIDE(i, n)
+    {
+        const double r = rho[i], uu = u[i], vv = v[i], ww = w[i];
+        W[i] = r;
+        W[n + i] = r * uu;
+        W[2 * n + i] = r * vv;
+        W[3 * n + i] = r * ww;
+        W[4 * n + i] = r * (e[i] + 0.5 * (uu * uu + vv * vv + ww * ww));
+    }
+}
+
+// get_primitive (cgrid.F90:764-797) followed by post_bc (:813-820):
+// p from (rho, e), then e and T from (rho, p) (IdealGasEOS.F90:45-70)
+__global__ __launch_bounds__(kBlock) void k_primitive(long long n, const double *__restrict__ W, double gam,
+                                                      double Rgas, double *__restrict__ rho, double *__restrict__ u,
+                                                      double *__restrict__ v, double *__restrict__ w,
+                                                      double *__restrict__ e, double *__restrict__ p,
+                                                      double *__restrict__ T)
+{
+    GRID_STRIDE(i, n)
+    {
+        const double r = W[i];
+        const double onebyrho = 1.0 / r;
+        const double uu = W[n + i] * onebyrho, vv = W[2 * n + i] * onebyrho, ww = W[3 * n + i] * onebyrho;
+        const double ee = W[4 * n + i] * onebyrho - 0.5 * (uu * uu + vv * vv + ww * ww);
+        const double pp = (gam - 1.0) * r * ee;
+        const double e2 = pp / (gam - 1.0) / r;
+        rho[i] = r;
+        u[i] = uu;
+        v[i] = vv;
+        w[i] = ww;
+        p[i] = pp;
+        e[i] = e2;
+        T[i] = e2 * ((gam - 1.0) / Rgas);
+    }
+}
+
+// LAD shear-viscosity sensor: magnitude of the strain rate (cgrid.F90:1085-1087);
+// du planes: du/dx du/dy du/dz dv/dx ... dw/dz
+__global__ __launch_bounds__(kBlock) void k_strain(long long n, const double *__restrict__ du,
+                                                   double *__restrict__ func, double *__restrict__ dil)
+{
+    GRID_STRIDE(i, n)
+    {
+        const double ux = du[i], uy = du[n + i], uz = du[2 * n + i];
+        const double vx = du[3 * n + i], vy = du[4 * n + i], vz = du[5 * n + i];
+        const double wx = du[6 * n + i], wy = du[7 * n + i], wz = du[8 * n + i];
+        func[i] = sqrt(ux * ux + 0.5 * (vx + uy) * (vx + uy) + 0.5 * (wx + uz) * (wx + uz) + vy * vy +
+                       0.5 * (wy + vz) * (wy + vz) + wz * wz);
+        dil[i] = ux + vy + wz;
+    }
+}
+
+// mustar = Cmu rho |sum of h^6-weighted fourth derivatives| (cgrid.F90:1110)
+__global__ __launch_bounds__(kBlock) void k_lad_mu(long long n, double Cmu, const double *__restrict__ rho,
+                                                   double *__restrict__ mustar)
+{
+    GRID_STRIDE(i, n) { mustar[i] = Cmu * rho[i] * fabs(mustar[i]); }
+}
+
+// artificial bulk viscosity (cgrid.F90:1115-1158): g = grad(rho) planes,
+// d4 = fourth derivatives of the dilatation along x, y, z
+__global__ __launch_bounds__(kBlock) void k_lad_bulk(long long n, double Cbeta, double hx, double hy, double hz,
+                                                     const double *__restrict__ rho, const double *__restrict__ g,
+                                                     const double *__restrict__ d4, const double *__restrict__ du,
+                                                     const double *__restrict__ dil, double *__restrict__ bulk)
+{
+    const double hx4 = hx * hx * hx * hx, hy4 = hy * hy * hy * hy, hz4 = hz * hz * hz * hz;
+    GRID_STRIDE(i, n)
+    {
+        const double g1 = g[i] * g[i], g2 = g[n + i] * g[n + i], g3 = g[2 * n + i] * g[2 * n + i];
+        const double den = g1 + g2 + g3 + 1.0e-32;
+        const double ax = hx * g1 / den, ay = hy * g2 / den, az = hz * g3 / den;
+        double b = d4[i] * hx4 * (ax * ax);
+        b = b + d4[2 * n + i] * hz4 * (az * az);
+        b = b + d4[n + i] * hy4 * (ay * ay);
+        const double uy = du[n + i], uz = du[2 * n + i], vx = du[3 * n + i], vz = du[5 * n + i];
+        const double wx = du[6 * n + i], wy = du[7 * n + i];
+        const double c1 = wy - vz, c2 = uz - wx, c3 = vx - uy;
+        const double curl2 = c1 * c1 + c2 * c2 + c3 * c3;
+        const double f = dil[i];
+        const double d2 = f * f;
+        const double fsw = f >= 0.0 ? 0.0 : d2 / (d2 + curl2 + 1.0e-32);
+        bulk[i] = Cbeta * rho[i] * fsw * fabs(b);
+    }
+}
+
+// artificial conductivity (cgrid.F90:1163-1196): g = grad(e) planes, d4 =
+// fourth derivatives of e along x, y, z
+__global__ __launch_bounds__(kBlock) void k_lad_kap(long long n, double Ckap, double gam, double hx, double hy,
+                                                    double hz, const double *__restrict__ rho,
+                                                    const double *__restrict__ p, const double *__restrict__ T,
+                                                    const double *__restrict__ g, const double *__restrict__ d4,
+                                                    double *__restrict__ kap)
+{
+    const double hx4 = hx * hx * hx * hx, hy4 = hy * hy * hy * hy, hz4 = hz * hz * hz * hz;
+    GRID_STRIDE(i, n)
+    {
+        const double g1 = g[i] * g[i], g2 = g[n + i] * g[n + i], g3 = g[2 * n + i] * g[2 * n + i];
+        const double den = g1 + g2 + g3 + 1.0e-32;
+        double k = d4[i] * hx4 * (hx * g1 / den);
+        k = k + d4[2 * n + i] * hz4 * (hz * g3 / den);
+        k = k + d4[n + i] * hy4 * (hy * g2 / den);
+        const double sos = sqrt(gam * p[i] / rho[i]);
+        kap[i] = Ckap * rho[i] * sos * fabs(k) / T[i];
+    }
+}
+
+// get_tau (cgrid.F90:1338-1390), get_q (:1392-1444) and the fluxes of
+// getRHS_x/y/z (:898-1056): f planes x (5), y (5), z (5); gT = grad(T)
+__global__ __launch_bounds__(kBlock) void k_fluxes(long long n, const double *__restrict__ W,
+                                                   const double *__restrict__ u, const double *__restrict__ v,
+                                                   const double *__restrict__ w, const double *__restrict__ p,
+                                                   const double *__restrict__ du, const double *__restrict__ mu,
+                                                   const double *__restrict__ bulk, const double *__restrict__ kap,
+                                                   const double *__restrict__ gT, double *__restrict__ f)
+{
+    GRID_STRIDE(i, n)
+    {
+        const double ux = du[i], uy = du[n + i], uz = du[2 * n + i];
+        const double vx = du[3 * n + i], vy = du[4 * n + i], vz = du[5 * n + i];
+        const double wx = du[6 * n + i], wy = du[7 * n + i], wz = du[8 * n + i];
+        const double m = mu[i], bk = bulk[i], kp = kap[i];
+        const double bambda = (4.0 / 3.0) * m + bk;
+        const double lam = bk - (2.0 / 3.0) * m;
+        const double txy = m * (uy + vx), txz = m * (uz + wx), tyz = m * (vz + wy);
+        const double txx = bambda * ux + lam * (vy + wz);
+        const double tyy = bambda * vy + lam * (ux + wz);
+        const double tzz = bambda * wz + lam * (ux + vy);
+        const double qx = -kp * gT[i], qy = -kp * gT[n + i], qz = -kp * gT[2 * n + i];
+        const double ru = W[n + i], rv = W[2 * n + i], rw = W[3 * n + i], TE = W[4 * n + i];
+        const double uu = u[i], vv = v[i], ww = w[i], pp = p[i];
+        f[i] = ru;
+        f[n + i] = ru * uu + pp - txx;
+        f[2 * n + i] = ru * vv - txy;
+        f[3 * n + i] = ru * ww - txz;
+        f[4 * n + i] = (TE + pp - txx) * uu - vv * txy - ww * txz + qx;
+        f[5 * n + i] = rv;
+        f[6 * n + i] = rv * uu - txy;
+        f[7 * n + i] = rv * vv + pp - tyy;
+        f[8 * n + i] = rv * ww - tyz;
+        f[9 * n + i] = (TE + pp - tyy) * vv - uu * txy - ww * tyz + qy;
+        f[10 * n + i] = rw;
+        f[11 * n + i] = rw * uu - txz;
+        f[12 * n + i] = rw * vv - tyz;
+        f[13 * n + i] = rw * ww + pp - tzz;
+        f[14 * n + i] = (TE + pp - tzz) * ww - uu * txz - vv * tyz + qz;
+    }
+}
+
+// low-storage RK45 stage (cgrid.F90:697-699) over the five conserved planes
+__global__ __launch_bounds__(kBlock) void k_rk(long long n5, double dt, double A, double B,
+                                               const double *__restrict__ rhs, double *__restrict__ Q,
+                                               double *__restrict__ W)
+{
+    GRID_STRIDE(i, n5)
+    {
+        const double q = dt * rhs[i] + A * Q[i];
+        Q[i] = q;
+        W[i] = W[i] + B * q;
+    }
+}
+
+int launched()
+{
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : 1000 + int(e);
+}
+
+}  // namespace
+}  // namespace pade
+
+using namespace pade;
+
+extern "C" {
+
+int pade_cg_conserved(long long n, const double *rho, const double *u, const double *v, const double *w,
+                      const double *e, double *W, void *stream)
+{
+    if (n < 1 || !rho || !u || !v || !w || !e || !W) return 400;
+    hipLaunchKernelGGL(k_conserved, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, n, rho, u, v, w, e, W);
+    return launched();
+}
+
+int pade_cg_primitive(long long n, const double *W, double gam, double Rgas, double *rho, double *u, double *v,
+                      double *w, double *e, double *p, double *T, void *stream)
+{
+    if (n < 1 || !W || !rho || !u || !v || !w || !e || !p || !T) return 400;
+    hipLaunchKernelGGL(k_primitive, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, n, W, gam, Rgas, rho,
+                       u, v, w, e, p, T);
+    return launched();
+}
+
+int pade_cg_strain(long long n, const double *du, double *func, double *dil, void *stream)
+{
+    if (n < 1 || !du || !func || !dil) return 400;
+    hipLaunchKernelGGL(k_strain, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, n, du, func, dil);
+    return launched();
+}
+
+int pade_cg_lad_mu(long long n, double Cmu, const double *rho, double *mustar, void *stream)
+{
+    if (n < 1 || !rho || !mustar) return 400;
+    hipLaunchKernelGGL(k_lad_mu, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, n, Cmu, rho, mustar);
+    return launched();
+}
+
+int pade_cg_lad_bulk(long long n, double Cbeta, double hx, double hy, double hz, const double *rho,
+                     const double *grad_rho, const double *d4, const double *du, const double *dil, double *bulk,
+                     void *stream)
+{
+    if (n < 1 || !rho || !grad_rho || !d4 || !du || !dil || !bulk) return 400;
+    hipLaunchKernelGGL(k_lad_bulk, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, n, Cbeta, hx, hy, hz,
+                       rho, grad_rho, d4, du, dil, bulk);
+    return launched();
+}
+
+int pade_cg_lad_kap(long long n, double Ckap, double gam, double hx, double hy, double hz, const double *rho,
+                    const double *p, const double *T, const double *grad_e, const double *d4, double *kap,
+                    void *stream)
+{
+    if (n < 1 || !rho || !p || !T || !grad_e || !d4 || !kap) return 400;
+    hipLaunchKernelGGL(k_lad_kap, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, n, Ckap, gam, hx, hy, hz,
+                       rho, p, T, grad_e, d4, kap);
+    return launched();
+}
+
+int pade_cg_fluxes(long long n, const double *W, const double *u, const double *v, const double *w, const double *p,
+                   const double *du, const double *mu, const double *bulk, const double *kap, const double *grad_T,
+                   double *fluxes, void *stream)
+{
+    if (n < 1 || !W || !u || !v || !w || !p || !du || !mu || !bulk || !kap || !grad_T || !fluxes) return 400;
+    hipLaunchKernelGGL(k_fluxes, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, n, W, u, v, w, p, du, mu,
+                       bulk, kap, grad_T, fluxes);
+    return launched();
+}
+
+int pade_cg_rk_stage(long long n, double dt, double A, double B, const double *rhs, double *Q, double *W, void *stream)
+{
+    if (n < 1 || !rhs || !Q || !W) return 400;
+    hipLaunchKernelGGL(k_rk, dim3(grid_for(5 * n)), dim3(kBlock), 0, (hipStream_t)stream, 5 * n, dt, A, B, rhs, Q, W);
+    return launched();
+}
+
+}  // extern "C"

@@ -1,12 +1,14 @@
 ! Drop-in for the reference's filters facade (module FiltersMod,
 ! src/filters/filters.F90:13-322): per-axis filter selection at init, then
 ! filterx/filtery/filterz on whole arrays (serial init, filters.F90:299-322).
-! The gaussian and least-squares filters are outside this library and stop
+! The periodic Gaussian filter is provided (gaussianstuff); the bounded
+! Gaussian and the least-squares filters are outside this library and stop
 ! with 52.
 module FiltersMod
     use iso_c_binding
     use padeops_c
     use cf90stuff, only: cf90
+    use gaussianstuff, only: gaussian
     implicit none
     private
     public :: filters
@@ -15,8 +17,9 @@ module FiltersMod
 
     type filters
         private
-        integer :: xmethod = 0, ymethod = 0, zmethod = 0    ! 1 cf90
+        integer :: xmethod = 0, ymethod = 0, zmethod = 0    ! 1 cf90, 2 gaussian
         type(cf90), allocatable :: xcf90, ycf90, zcf90
+        type(gaussian), allocatable :: xgau, ygau, zgau
         integer, dimension(3) :: xsz = 0, ysz = 0, zsz = 0
         logical :: initialized = .false.
     contains
@@ -37,6 +40,7 @@ contains
         character(len=pade_clen) :: s
         s = ""
         if (m == 1) s = "cf90"
+        if (m == 2) s = "gaussian"
     end function
 
     function getMethodx(this) result(m)
@@ -70,24 +74,31 @@ contains
         this%ysz = this%xsz
         this%zsz = this%xsz
         ! filters.F90:82-184
-        call setup(methodx, nx, periodicx, "X", this%xmethod, this%xcf90)
-        call setup(methody, ny, periodicy, "Y", this%ymethod, this%ycf90)
-        call setup(methodz, nz, periodicz, "Z", this%zmethod, this%zcf90)
+        call setup(methodx, nx, periodicx, "X", this%xmethod, this%xcf90, this%xgau)
+        call setup(methody, ny, periodicy, "Y", this%ymethod, this%ycf90, this%ygau)
+        call setup(methodz, nz, periodicz, "Z", this%zmethod, this%zcf90, this%zgau)
         this%initialized = .true.
     end subroutine
 
-    subroutine setup(method, n, periodic, axis, m, o)
+    subroutine setup(method, n, periodic, axis, m, o, g)
         character(len=*), intent(in) :: method, axis
         integer, intent(in) :: n
         logical, intent(in) :: periodic
         integer, intent(out) :: m
         type(cf90), allocatable, intent(inout) :: o
+        type(gaussian), allocatable, intent(inout) :: g
         select case (method)
         case ("cf90")
             allocate(o)
             if (o%init(n, periodic) /= 0) call pade_fail("Initializing cf90 failed in " // axis // " ", 51)
             m = 1
-        case ("gaussian", "lstsq")
+        case ("gaussian")
+            if (.not. periodic) &
+                call pade_fail("the bounded gaussian filter in " // axis // " is not provided by libpadeops_hip", 52)
+            allocate(g)
+            if (g%init(n, periodic) /= 0) call pade_fail("Initializing gaussian filter failed in " // axis // " ", 51)
+            m = 2
+        case ("lstsq")
             call pade_fail("the " // method // " filter in " // axis // " is not provided by libpadeops_hip", 52)
         case default
             call pade_fail("Incorrect method select in direction " // axis, 52)
@@ -110,6 +121,18 @@ contains
             call this%zcf90%destroy()
             deallocate(this%zcf90)
         end if
+        if (allocated(this%xgau)) then
+            call this%xgau%destroy()
+            deallocate(this%xgau)
+        end if
+        if (allocated(this%ygau)) then
+            call this%ygau%destroy()
+            deallocate(this%ygau)
+        end if
+        if (allocated(this%zgau)) then
+            call this%zgau%destroy()
+            deallocate(this%zgau)
+        end if
         this%xmethod = 0
         this%ymethod = 0
         this%zmethod = 0
@@ -122,8 +145,14 @@ contains
         real(rkind), dimension(this%xsz(1), this%xsz(2), this%xsz(3)), intent(in) :: f
         real(rkind), dimension(this%xsz(1), this%xsz(2), this%xsz(3)), intent(out) :: ff
         integer, optional, intent(in) :: bc1, bcn
-        if (this%xmethod /= 1) call pade_fail("filters used before init", 400)
-        call this%xcf90%filter1(f, ff, this%xsz(2), this%xsz(3), bc1, bcn)
+        select case (this%xmethod)
+        case (1)
+            call this%xcf90%filter1(f, ff, this%xsz(2), this%xsz(3), bc1, bcn)
+        case (2)
+            call this%xgau%filter1(f, ff, this%xsz(2), this%xsz(3), bc1, bcn)
+        case default
+            call pade_fail("filters used before init", 400)
+        end select
     end subroutine
 
     ! filters.F90:237-252
@@ -132,8 +161,14 @@ contains
         real(rkind), dimension(this%ysz(1), this%ysz(2), this%ysz(3)), intent(in) :: f
         real(rkind), dimension(this%ysz(1), this%ysz(2), this%ysz(3)), intent(out) :: ff
         integer, optional, intent(in) :: bc1, bcn
-        if (this%ymethod /= 1) call pade_fail("filters used before init", 400)
-        call this%ycf90%filter2(f, ff, this%ysz(1), this%ysz(3), bc1, bcn)
+        select case (this%ymethod)
+        case (1)
+            call this%ycf90%filter2(f, ff, this%ysz(1), this%ysz(3), bc1, bcn)
+        case (2)
+            call this%ygau%filter2(f, ff, this%ysz(1), this%ysz(3), bc1, bcn)
+        case default
+            call pade_fail("filters used before init", 400)
+        end select
     end subroutine
 
     ! filters.F90:254-269
@@ -142,8 +177,14 @@ contains
         real(rkind), dimension(this%zsz(1), this%zsz(2), this%zsz(3)), intent(in) :: f
         real(rkind), dimension(this%zsz(1), this%zsz(2), this%zsz(3)), intent(out) :: ff
         integer, optional, intent(in) :: bc1, bcn
-        if (this%zmethod /= 1) call pade_fail("filters used before init", 400)
-        call this%zcf90%filter3(f, ff, this%zsz(1), this%zsz(2), bc1, bcn)
+        select case (this%zmethod)
+        case (1)
+            call this%zcf90%filter3(f, ff, this%zsz(1), this%zsz(2), bc1, bcn)
+        case (2)
+            call this%zgau%filter3(f, ff, this%zsz(1), this%zsz(2), bc1, bcn)
+        case default
+            call pade_fail("filters used before init", 400)
+        end select
     end subroutine
 
 end module FiltersMod

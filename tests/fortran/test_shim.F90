@@ -117,6 +117,14 @@ program test_shim
     call fil%init(n1, n2, n3, .true., .true., .false., "cf90", "cf90", "cf90")
     call fil%filterz(f, g)
     call put("fil_filterz", g)
+    ! the periodic Gaussian filter through the facade (filters.F90:103-108)
+    call fil%init(n1, n2, n3, .true., .true., .true., "gaussian", "gaussian", "gaussian")
+    call fil%filterx(f, g)
+    call put("gau_filterx", g)
+    call fil%filtery(f, g)
+    call put("gau_filtery", g)
+    call fil%filterz(f, g)
+    call put("gau_filterz", g)
 
     ! staggered z operators, periodic (cd06stagg.F90): E arrays of n3 + 1 planes
     allocate(fe(n1, n2, n3 + 1), ge(n1, n2, n3 + 1), fce(n1, n2, n3 + 1), gcc(n1, n2, n3))

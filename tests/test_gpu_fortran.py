@@ -113,6 +113,18 @@ def test_fortran_cd06stagg(records):
         assert np.abs(got - expect).max() / np.abs(expect).max() < 1e-12, tag
 
 
+def test_fortran_gaussian_filter(records):
+    """The periodic Gaussian filter through the Fortran filters facade
+    (gaussianstuff, src/filters/gaussian.F90)."""
+    f = records["input"]
+    for axis, tag in enumerate(("gau_filterx", "gau_filtery", "gau_filterz")):
+        n = f.shape[axis]
+        G = pd.dense_operator("gaussian", "filter", n, 1.0, True)
+        expect = np.moveaxis(np.tensordot(G, f, axes=([1], [axis])), 0, axis)
+        got = records[tag]
+        assert np.abs(got - expect).max() / np.abs(expect).max() < 1e-12, tag
+
+
 def test_fortran_cd06stagg_bounded(records):
     """Bounded staggered operators through cd06staggstuff's init_nonperiodic
     (keyword arguments), ddz_C2C, ddz_E2E and the complex ddz_E2C."""

@@ -132,3 +132,18 @@ def test_planner_gaussian_periodic(n, M):
     meta = (ctypes.c_int * 32)()
     lib = _lib.load()
     assert lib.pade_plan_export(5, 3, 32, 1.0, 0, 0, 0, 0, 0, None, 0, meta) == 21   # bounded: not provided
+
+
+def test_compressible_kernels_reject_bad_arguments():
+    """The compressible-step entry points check their arguments on the host
+    (no GPU call): empty fields or null arrays give 400."""
+    lib = _lib.load()
+    nul = None
+    assert lib.pade_cg_conserved(0, nul, nul, nul, nul, nul, nul, nul) == 400
+    assert lib.pade_cg_primitive(8, nul, 1.4, 1.0, nul, nul, nul, nul, nul, nul, nul, nul) == 400
+    assert lib.pade_cg_strain(8, nul, nul, nul, nul) == 400
+    assert lib.pade_cg_lad_mu(-1, 0.002, nul, nul, nul) == 400
+    assert lib.pade_cg_lad_bulk(8, 1.75, 0.1, 0.1, 0.1, nul, nul, nul, nul, nul, nul, nul) == 400
+    assert lib.pade_cg_lad_kap(8, 0.01, 1.4, 0.1, 0.1, 0.1, nul, nul, nul, nul, nul, nul, nul) == 400
+    assert lib.pade_cg_fluxes(8, nul, nul, nul, nul, nul, nul, nul, nul, nul, nul, nul, nul) == 400
+    assert lib.pade_cg_rk_stage(8, 1e-3, 0.0, 0.1, nul, nul, nul, nul) == 400

@@ -204,7 +204,9 @@ int pade_plan_export(int kind, int oper, int n, double dx, int periodic, int bc1
  * variables (rho, rho u, rho v, rho w, total energy), du the nine velocity
  * gradients (du/dx, du/dy, du/dz, dv/dx, ..., dw/dz), grad_* three gradient
  * components, d4 three fourth derivatives (x, y, z), fluxes the fifteen
- * x / y / z fluxes of the five equations.  The derivatives and filters
+ * x / y / z fluxes of the five equations (the x fluxes negated, so that the
+ * first x derivative writes the right-hand side directly, -d(fx)/dx).  The
+ * derivatives and filters
  * between them are the operator calls above.  Return 0 or 400 / 1000 + hip
  * error. */
 int pade_cg_conserved(long long n, const double *rho, const double *u, const double *v, const double *w,

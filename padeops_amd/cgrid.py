@@ -235,10 +235,11 @@ class cgrid:
         _lib.check(lib.pade_cg_fluxes(N, _p(self.W), _p(self.u), _p(self.v), _p(self.w), _p(self.p), _p(du),
                                       _p(self.mu), _p(self.bulk), _p(self.kap), _p(gT), _p(fl), st))
         del du, gT
-        # rhs = 0; rhs -= d(flux)/dx, then y, then z (getRHS_x, _y, _z), each
-        # difference accumulated inside the derivative kernel
-        rhs.zero_()
-        for a in (0, 1, 2):
+        # rhs = -d(fx)/dx (the x fluxes come negated), then rhs -= d(fy)/dy,
+        # d(fz)/dz (getRHS_x, _y, _z), accumulated inside the derivative kernel
+        for k in range(5):
+            self._op(PADE_D1, 0, fl[:, :, :, k], rhs[:, :, :, k])
+        for a in (1, 2):
             for k in range(5):
                 self._op(PADE_D1, a, fl[:, :, :, 5 * a + k], rhs[:, :, :, k], alpha=-1.0, beta=1.0)
 

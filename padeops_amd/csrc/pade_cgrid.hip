@@ -141,7 +141,10 @@ __global__ __launch_bounds__(kBlock) void k_lad_kap(long long n, double Ckap, do
 }
 
 // get_tau (cgrid.F90:1338-1390), get_q (:1392-1444) and the fluxes of
-// getRHS_x/y/z (:898-1056): f planes x (5), y (5), z (5); gT = grad(T)
+// getRHS_x/y/z (:898-1056): f planes x (5), y (5), z (5); gT = grad(T).
+// The x fluxes are stored negated: the right-hand side starts as d(-fx)/dx,
+// which equals 0 - d(fx)/dx bit for bit (the operator is linear and
+// negation exact), so it needs neither a cleared array nor an accumulate.
 __global__ __launch_bounds__(kBlock) void k_fluxes(long long n, const double *__restrict__ W,
                                                    const double *__restrict__ u, const double *__restrict__ v,
                                                    const double *__restrict__ w, const double *__restrict__ p,
@@ -164,11 +167,11 @@ __global__ __launch_bounds__(kBlock) void k_fluxes(long long n, const double *__
         const double qx = -kp * gT[i], qy = -kp * gT[n + i], qz = -kp * gT[2 * n + i];
         const double ru = W[n + i], rv = W[2 * n + i], rw = W[3 * n + i], TE = W[4 * n + i];
         const double uu = u[i], vv = v[i], ww = w[i], pp = p[i];
-        f[i] = ru;
-        f[n + i] = ru * uu + pp - txx;
-        f[2 * n + i] = ru * vv - txy;
-        f[3 * n + i] = ru * ww - txz;
-        f[4 * n + i] = (TE + pp - txx) * uu - vv * txy - ww * txz + qx;
+        f[i] = -ru;
+        f[n + i] = -(ru * uu + pp - txx);
+        f[2 * n + i] = -(ru * vv - txy);
+        f[3 * n + i] = -(ru * ww - txz);
+        f[4 * n + i] = -((TE + pp - txx) * uu - vv * txy - ww * txz + qx);
         f[5 * n + i] = rv;
         f[6 * n + i] = rv * uu - txy;
         f[7 * n + i] = rv * vv + pp - tyy;

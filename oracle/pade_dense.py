@@ -25,6 +25,9 @@ Pinning (tests/test_oracle_*.py):
     with respect to reference output),
   * periodic staggered operators (``src/derivatives/cd06stagg.F90``) -> sixth
     order of accuracy on smooth periodic data at edges and cells,
+  * bounded staggered operators (``STAGG_CD06_files/``) -> even / odd ends
+    equal the periodic operator on the mirror-extended data (period 4n);
+    one-sided ends exact on polynomials to the degree of their construction,
   * periodic Gaussian filter (``src/filters/gaussian.F90``) -> its transfer
     function agf + 2 sum_m c_m cos(m k) on every Fourier mode,
   * complex z filter (``src/filters/cf90.F90:1230-1251``) -> composition of

@@ -35,7 +35,8 @@ extern "C" {
 #endif
 
 /* PADE_GAUSSIAN: the explicit Gaussian filter of width 4 dx
- * (src/filters/gaussian.F90:11-17,137-185), periodic form, n = 1 or n >= 9;
+ * (src/filters/gaussian.F90:11-36, periodic stencil :137-185, bounded rows
+ * :189-334 with the apply-time bc codes 0 / 1 / -1), n = 1 or n >= 9;
  * PADE_FILTER on any axis, dx ignored.  It is the compressible solver's
  * filter of its artificial transport properties (cgrid.F90:338,1113). */
 enum pade_kind { PADE_CD06 = 1, PADE_CD10 = 2, PADE_CF90 = 3, PADE_CD06STAGG = 4, PADE_GAUSSIAN = 5 };

@@ -351,7 +351,31 @@ def _left_rows_cf90(bc1):
     raise ValueError("bc must be -1, 0 or 1")
 
 
+def _left_rows_gaussian(bc1):
+    """Bounded Gaussian filter, src/filters/gaussian.F90 (A = I): bc 0 rows
+    :192-208 with the closure weights :19-36; bc 1 rows :209-231; bc -1 rows
+    :232-254."""
+    a, b, c, d, e = gaussian_coeffs()
+    if bc1 == 0:
+        B = [(0, _acc((5.0 / 6.0, 1), (1.0 / 6.0, 2))),
+             (1, _acc((2.0 / 3.0, 2), (1.0 / 6.0, 3), (1.0 / 6.0, 1))),
+             (2, _acc((31.0 / 64.0, 3), (7.0 / 32.0, 4), (7.0 / 32.0, 2), (5.0 / 128.0, 5), (5.0 / 128.0, 1))),
+             (3, _acc((17.0 / 48.0, 4), (15.0 / 64.0, 5), (15.0 / 64.0, 3), (7.0 / 96.0, 6), (7.0 / 96.0, 2),
+                      (1.0 / 64.0, 7), (1.0 / 64.0, 1)))]
+        return [], B, 1
+    if bc1 in (1, -1):
+        s = float(bc1)
+        B = [(0, _acc((a, 1), (b, 2), (s * b, 2), (c, 3), (s * c, 3), (d, 4), (s * d, 4), (e, 5), (s * e, 5))),
+             (1, _acc((a, 2), (b, 3), (b, 1), (c, 4), (s * c, 2), (d, 5), (s * d, 3), (e, 6), (s * e, 4))),
+             (2, _acc((a, 3), (b, 4), (b, 2), (c, 5), (c, 1), (d, 6), (s * d, 2), (e, 7), (s * e, 3))),
+             (3, _acc((a, 4), (b, 5), (b, 3), (c, 6), (c, 2), (d, 7), (d, 1), (e, 8), (s * e, 2)))]
+        return [], B, 1
+    raise ValueError("bc must be -1, 0 or 1")
+
+
 def _left_rows(kind, which, bc):
+    if kind == "gaussian" and which == "filter":
+        return _left_rows_gaussian(bc)
     if kind == "cd06" and which == "d1":
         return _left_rows_cd06_d1(bc)
     if kind == "cd10" and which == "d1":

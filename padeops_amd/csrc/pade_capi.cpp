@@ -531,8 +531,6 @@ int pade_op_create(int kind, int n, double dx, int periodic, int bc1, int bcn, p
     if (n < 1) return fail(400, "n must be positive");
     const bool filter_kind = kind == pade::CF90 || kind == pade::GAUSSIAN;   // dx unused
     if (!filter_kind && !(dx > 0.0) && n > 1) return fail(400, "dx must be positive");
-    if (kind == pade::GAUSSIAN && !periodic && n > 1)
-        return fail(21, "the Gaussian filter is provided in its periodic form");
     if (int ierr = pade::init_error(kind, n, periodic != 0))
         return fail(ierr, "grid too small for this operator");
     if (kind == pade::CD06STAGG && !periodic) {

@@ -77,8 +77,8 @@ Interior interior(int kind, int oper)
         s.b[0] = s.b[8] = -5.0e-6;
         s.H = 4; s.parity = 1; s.dxpow = 0;
     } else if (kind == GAUSSIAN && oper == FILTER) {
-        // explicit Gaussian filter of width 4 dx, periodic form
-        // (src/filters/gaussian.F90:11-17 coefficients, :137-185 stencil): A = I
+        // explicit Gaussian filter of width 4 dx (src/filters/gaussian.F90:11-17
+        // coefficients, :137-185 periodic stencil, bounded rows below): A = I
         s.a[2] = 1.0;
         s.b[4] = 3565.0 / 10368.0;
         s.b[3] = s.b[5] = 3091.0 / 12960.0;
@@ -202,6 +202,14 @@ void closure_rows(int kind, int oper, std::vector<Row> &A, std::vector<Row> &B)
         B.push_back({2, {{2, 9.9952e-1}, {3, 6.6656e-1}, {1, 6.6656e-1}, {4, 1.668e-1}, {0, 1.668e-1}}});
         B.push_back({3, {{3, 9.9968e-1}, {4, 6.6652e-1}, {2, 6.6652e-1}, {5, 1.6672e-1},
                          {1, 1.6672e-1}, {6, 4.0e-5}, {0, 4.0e-5}}});
+    } else if (kind == GAUSSIAN && oper == FILTER) {
+        // explicit closure rows of the bounded Gaussian filter (A = I),
+        // gaussian.F90:19-36 weights, rows :192-208
+        B.push_back({0, {{0, 5.0 / 6.0}, {1, 1.0 / 6.0}}});
+        B.push_back({1, {{1, 2.0 / 3.0}, {2, 1.0 / 6.0}, {0, 1.0 / 6.0}}});
+        B.push_back({2, {{2, 31.0 / 64.0}, {3, 7.0 / 32.0}, {1, 7.0 / 32.0}, {4, 5.0 / 128.0}, {0, 5.0 / 128.0}}});
+        B.push_back({3, {{3, 17.0 / 48.0}, {4, 15.0 / 64.0}, {2, 15.0 / 64.0}, {5, 7.0 / 96.0}, {1, 7.0 / 96.0},
+                         {6, 1.0 / 64.0}, {0, 1.0 / 64.0}}});
     }
 }
 
@@ -502,8 +510,9 @@ int init_error(int kind, int n, bool periodic)
     if (kind == CD06) return periodic ? (n >= 6 ? 0 : 3) : (n >= 8 ? 0 : 2);
     if (kind == CD10) return n >= 8 ? 0 : 2;
     if (kind == CF90) return n >= 10 ? 0 : 7;
-    // the periodic Gaussian stencil spans 9 points (gaussian.F90:137-185)
-    if (kind == GAUSSIAN) return periodic && n >= 9 ? 0 : 21;
+    // the Gaussian stencil spans 9 points (gaussian.F90:137-185); the
+    // reference's init checks nothing and would index out of its array below
+    if (kind == GAUSSIAN) return n >= 9 ? 0 : 21;
     // cd06stagg.F90:182-184: GracefulExit 21 for nx <= 4
     if (kind == CD06STAGG) return n > 4 ? 0 : 21;
     return 1;

@@ -14,8 +14,8 @@ MPI job) these classes raise :class:`PadeError` with the same code:
 11 / 12 cd10 / cd06 init failed, 21 cd06 second derivative, 4 / 5 / 6 metric
 or curvilinear terms requested, 1 invalid derivative method, 51 cf90 init
 failed, 52 invalid filter method.  The spectral ("four", "cheb") derivative
-methods, the least-squares filter and the bounded Gaussian filter are outside
-this library and raise 13 / 14 / 52; the periodic Gaussian filter is provided.
+methods and the least-squares filter are outside this library and raise
+13 / 14 / 52; the Gaussian filter is provided.
 """
 from __future__ import annotations
 
@@ -174,9 +174,8 @@ class filters:
             raise TypeError("init takes sizes, (periodicx, periodicy, periodicz) and (methodx, methody, methodz)")
         pers, methods = rest[0:3], rest[3:6]
         for axis, m, per in zip(_AXES, methods, pers):
-            if m == "lstsq" or (m == "gaussian" and not per):
-                raise PadeError(52, f"the {'bounded ' if m == 'gaussian' else ''}{m} filter in {axis} "
-                                    "is not provided by this compact-operator library")
+            if m == "lstsq":
+                raise PadeError(52, f"the {m} filter in {axis} is not provided by this compact-operator library")
             if m not in ("cf90", "gaussian"):
                 raise PadeError(52, f"Incorrect method select in direction {axis.upper()}")
         if self.initialized:

@@ -1,9 +1,8 @@
 ! Drop-in for the reference's filters facade (module FiltersMod,
 ! src/filters/filters.F90:13-322): per-axis filter selection at init, then
 ! filterx/filtery/filterz on whole arrays (serial init, filters.F90:299-322).
-! The periodic Gaussian filter is provided (gaussianstuff); the bounded
-! Gaussian and the least-squares filters are outside this library and stop
-! with 52.
+! The Gaussian filter is provided (gaussianstuff); the least-squares filter is
+! outside this library and stops with 52.
 module FiltersMod
     use iso_c_binding
     use padeops_c
@@ -93,8 +92,6 @@ contains
             if (o%init(n, periodic) /= 0) call pade_fail("Initializing cf90 failed in " // axis // " ", 51)
             m = 1
         case ("gaussian")
-            if (.not. periodic) &
-                call pade_fail("the bounded gaussian filter in " // axis // " is not provided by libpadeops_hip", 52)
             allocate(g)
             if (g%init(n, periodic) /= 0) call pade_fail("Initializing gaussian filter failed in " // axis // " ", 51)
             m = 2

@@ -1,8 +1,7 @@
 ! Drop-in for the reference's Gaussian filter module (src/filters/gaussian.F90):
-! same module, type and procedure names and argument orders; the periodic
-! filter runs in libpadeops_hip.so (kind PADE_GAUSSIAN).  The bounded form
-! (gaussian.F90:19-36 closure rows) is not provided: init on a bounded grid
-! stops with 52, the code the filters facade gives a filter it does not have.
+! same module, type and procedure names and argument orders; the filter,
+! periodic or bounded (bc 0 / 1 / -1 at apply time), runs in libpadeops_hip.so
+! (kind PADE_GAUSSIAN).
 module gaussianstuff
     use iso_c_binding
     use padeops_c
@@ -40,17 +39,16 @@ module gaussianstuff
 
 contains
 
-    ! gaussian.F90:67-91; the periodic stencil spans 9 points
+    ! gaussian.F90:67-91; the stencil spans 9 points (stops with 21 for
+    ! 1 < n < 9, where the reference would index outside its arrays)
     function init(this, n_, periodic_) result(ierr)
         class(gaussian), intent(inout) :: this
         integer, intent(in) :: n_
         logical, intent(in) :: periodic_
         integer :: ierr
         if (c_associated(this%h)) call this%destroy()
-        if (.not. periodic_ .and. n_ > 1) &
-            call pade_fail("the bounded gaussian filter is not provided by libpadeops_hip", 52)
-        call pade_check(pade_op_create(PADE_GAUSSIAN, int(n_, c_int), 1.0_c_double, 1_c_int, 0_c_int, 0_c_int, &
-                                       this%h), "gaussian%init")
+        call pade_check(pade_op_create(PADE_GAUSSIAN, int(n_, c_int), 1.0_c_double, merge(1_c_int, 0_c_int, periodic_), &
+                                       0_c_int, 0_c_int, this%h), "gaussian%init")
         this%n = n_
         ierr = 0
     end function

@@ -321,16 +321,14 @@ class cf90(_CompactOp):
 
 
 class gaussian(_CompactOp):
-    """Explicit Gaussian filter of width 4 dx, periodic form
-    (src/filters/gaussian.F90: coefficients :11-17, stencil :137-185); the
-    compressible solver's artificial-property filter.  The bounded form is
-    not provided."""
+    """Explicit Gaussian filter of width 4 dx (src/filters/gaussian.F90:
+    coefficients :11-36, periodic stencil :137-185, bounded rows for bc 0 /
+    1 / -1 :189-334); the compressible solver's artificial-property filter."""
 
     _kind = PADE_GAUSSIAN
 
     def init(self, n, periodic):
-        """Returns 0, or 21 for a bounded grid or 1 < n < 9 (the periodic
-        stencil spans 9 points)."""
+        """Returns 0, or 21 for 1 < n < 9 (the stencil spans 9 points)."""
         return self._create(n, 1.0, periodic, 0, 0)
 
     def filter1(self, f, df, na, nb, bc1=None, bcn=None):

@@ -20,12 +20,12 @@ def test_metric_flags(kw, code):
     assert e.value.code == code
 
 
-@pytest.mark.parametrize("methods,per", [(("gaussian", "cf90", "cf90"), (False, True, True)),
+@pytest.mark.parametrize("methods,per", [(("gaussian", "lstsq", "cf90"), (False, True, True)),
                                          (("cf90", "lstsq", "cf90"), (True, True, True)),
                                          (("cf90", "cf90", "box"), (True, True, True))])
 def test_filter_methods(methods, per):
-    # the periodic Gaussian filter is provided; the bounded one, the
-    # least-squares filter and unknown names stop with 52 (filters.F90:120)
+    # the Gaussian filter is provided; the least-squares filter and unknown
+    # names stop with 52 (filters.F90:120)
     with pytest.raises(po.PadeError) as e:
         po.filters().init(16, 16, 16, *per, *methods)
     assert e.value.code == 52

@@ -152,6 +152,19 @@ def test_gaussian_filter_c_abi(n):
         ref = np.moveaxis(np.tensordot(G, f, axes=([1], [axis - 1])), 0, axis - 1)
         assert relerr(out.cpu().numpy(), ref) < 1e-12, (n, axis)
     lib.pade_op_destroy(h)
-    # bounded form, and too short a periodic grid: 21
-    assert lib.pade_op_create(_lib.PADE_GAUSSIAN, 32, 0.0, 0, 0, 0, ctypes.byref(h)) == 21
+    # bounded form with each apply-time bc pair (gaussian.F90:189-334)
+    if n >= 9:
+        _lib.check(lib.pade_op_create(_lib.PADE_GAUSSIAN, n, 0.0, 0, 0, 0, ctypes.byref(h)))
+        f = np.asfortranarray(rng.standard_normal((5, 4, n)))
+        ft = _to_gpu(f)
+        out = po.fortran_empty((5, 4, n), device=DEV)
+        for b1, bn in ((0, 0), (1, -1), (-1, 1), (1, 1), (0, -1)):
+            Gb = pd.dense_operator("gaussian", "filter", n, 1.0, False, b1, bn)
+            _lib.check(lib.pade_op_apply(h, _lib.PADE_FILTER, 3, ctypes.c_void_p(ft.data_ptr()),
+                                         ctypes.c_void_p(out.data_ptr()), 5, 4, b1, bn, None))
+            torch.cuda.synchronize()
+            ref = np.moveaxis(np.tensordot(Gb, f, axes=([1], [2])), 0, 2)
+            assert relerr(out.cpu().numpy(), ref) < 1e-12, (n, b1, bn)
+        lib.pade_op_destroy(h)
+    # shorter than the 9-point stencil: 21
     assert lib.pade_op_create(_lib.PADE_GAUSSIAN, 8, 0.0, 1, 0, 0, ctypes.byref(h)) == 21

@@ -119,7 +119,7 @@ def _mirror_rows(kind, which, n, bc_left, bc_right):
     return A, B, H
 
 
-@pytest.mark.parametrize("kind,which", [("cd10", "d1"), ("cd10", "d2"), ("cf90", "filter")])
+@pytest.mark.parametrize("kind,which", [("cd10", "d1"), ("cd10", "d2"), ("cf90", "filter"), ("gaussian", "filter")])
 @pytest.mark.parametrize("bcl", [1, -1])
 @pytest.mark.parametrize("bcr", [1, -1])
 def test_symmetry_rows_are_mirror_extension(kind, which, bcl, bcr):
@@ -145,7 +145,7 @@ def test_one_sided_closures_are_conservative(kind):
 
 
 @pytest.mark.parametrize("kind,which,maxdeg", [("cd06", "d1", 4), ("cd10", "d1", 4), ("cd10", "d2", 4),
-                                               ("cf90", "filter", 1)])
+                                               ("cf90", "filter", 1), ("gaussian", "filter", 0)])
 def test_one_sided_closures_polynomial_exactness(kind, which, maxdeg):
     n = 24
     x = np.linspace(-1.0, 1.0, n)
@@ -253,3 +253,17 @@ def test_oracle_reproduces_golden_fixtures():
     for c in golden_cases.cases():
         y = pd.apply(c["kind"], c["which"], c["f"], c["axis"] - 1, c["dx"], c["periodic"], *c["bc"])
         assert np.abs(y - c["y"]).max() <= 1e-13 * np.abs(c["y"]).max(), c["name"]
+
+
+def test_gaussian_bounded_closure_rows_are_symmetric_filters():
+    """The bounded Gaussian filter's closure rows 2-4 (gaussian.F90:197-208)
+    are symmetric about their point with unit sum, so they keep linear data;
+    row 1 keeps constants only; the right end mirrors the left."""
+    n = 20
+    A, B = pd.assemble("gaussian", "filter", n, 1.0, False, 0, 0)
+    assert np.array_equal(A, np.eye(n))
+    x = np.arange(n, dtype=float)
+    for r in (1, 2, 3, n - 4, n - 3, n - 2):
+        assert abs(B[r] @ x - x[r]) < 1e-13, r
+    assert abs(B[0].sum() - 1.0) < 1e-15 and abs(B[0] @ x - x[0]) > 0.1
+    assert np.allclose(B[::-1, ::-1], B, atol=0)

@@ -120,18 +120,22 @@ def test_planner_stagg_bounded_codes():
 
 
 @pytest.mark.parametrize("n,M", [(64, 16), (64, 32), (512, 64), (37, 37), (9, 9)])
-def test_planner_gaussian_periodic(n, M):
-    """The periodic Gaussian filter (identity A, 9-point B) through the
-    exported tables, forward and adjoint (its own transpose)."""
+def test_planner_gaussian(n, M):
+    """The Gaussian filter (identity A, 9-point B), periodic and bounded with
+    every bc pair, through the exported tables, forward and adjoint."""
     v = np.random.default_rng(n).standard_normal(n)
-    G = pd.dense_operator("gaussian", "filter", n, 1.0, True)
-    for adj in (False, True):
-        out = se.apply_pencil("gaussian", "filter", v, 1.0, True, 0, 0, adjoint=adj, M=M)
-        ref = (G.T if adj else G) @ v
-        assert np.abs(out - ref).max() / np.abs(ref).max() < 1e-13
+    cases = [(True, 0, 0)] + [(False, a, b) for a in (0, 1, -1) for b in (0, 1, -1)]
+    for periodic, b1, bn in cases:
+        G = pd.dense_operator("gaussian", "filter", n, 1.0, periodic, b1, bn)
+        for adj in (False, True):
+            if adj and not periodic and n < 16:
+                continue                                     # bounded adjoints need n >= 16
+            out = se.apply_pencil("gaussian", "filter", v, 1.0, periodic, b1, bn, adjoint=adj, M=M)
+            ref = (G.T if adj else G) @ v
+            assert np.abs(out - ref).max() / np.abs(ref).max() < 1e-13, (periodic, b1, bn, adj)
     meta = (ctypes.c_int * 32)()
     lib = _lib.load()
-    assert lib.pade_plan_export(5, 3, 32, 1.0, 0, 0, 0, 0, 0, None, 0, meta) == 21   # bounded: not provided
+    assert lib.pade_plan_export(5, 3, 8, 1.0, 1, 0, 0, 0, 0, None, 0, meta) == 21   # shorter than the stencil
 
 
 def test_compressible_kernels_reject_bad_arguments():

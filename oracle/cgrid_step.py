@@ -1,6 +1,7 @@
 """CPU oracle of the compressible solver's Runge-Kutta step for one ideal-gas
-species on a periodic grid (src/compressible/cgrid.F90 with IdealGasEOS.F90,
-PowerLawViscosity.F90 and MixtureEOS.F90 at ns = 1).
+species (src/compressible/cgrid.F90 with IdealGasEOS.F90,
+PowerLawViscosity.F90 and MixtureEOS.F90 at ns = 1), periodic or bounded per
+axis with the reference's bc pairs and their sign flips.
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the product may import this module;
 only tests/ use it, as the checker.
@@ -14,7 +15,9 @@ Pinning (tests/test_oracle_cgrid.py): the RK45 coefficients by the order
 conditions of the low-storage scheme; the inviscid and viscous fluxes
 against the compressible Navier-Stokes equations evaluated with FFT
 derivatives; conservation of mass, momentum and energy by the full step;
-invariance of a uniform state.  The artificial-property (LAD) formulas are
+invariance of a uniform state; on bounded axes with the symmetric pair, the
+step equals the periodic step of the mirror-extended state on its half (pins
+every bc sign flip).  The artificial-property (LAD) formulas are
 restated without an independent closed form ("parity unpinned" beyond
 conservation and the uniform-state check).
 """
@@ -33,38 +36,53 @@ RK45_B = (494393426753.0 / 4806282396855.0, 4047970641027.0 / 5463924506627.0,
 
 class Grid:
     def __init__(self, shape, spacing, derivative="cd10", filt="cf90", gam=1.4, Rgas=1.0, Cmu=0.002,
-                 Cbeta=1.75, Ckap=0.01, inviscid=True, mu_ref=1.0, T_ref=1.0, visc_n=0.75, Pr=0.75):
+                 Cbeta=1.75, Ckap=0.01, inviscid=True, mu_ref=1.0, T_ref=1.0, visc_n=0.75, Pr=0.75,
+                 periodic=(True, True, True), bc=((0, 0), (0, 0), (0, 0))):
         self.shape, self.h = tuple(shape), tuple(spacing)
         self.gam, self.Rgas = gam, Rgas
         self.Cmu, self.Cbeta, self.Ckap = Cmu, Cbeta, Ckap
         self.inviscid = inviscid
         self.mu_ref, self.T_ref, self.visc_n, self.Pr = mu_ref, T_ref, visc_n, Pr
-        self.D1 = [pd.dense_operator(derivative, "d1", n, h, True) for n, h in zip(shape, spacing)]
-        self.D2 = [pd.dense_operator(derivative, "d2", n, h, True) for n, h in zip(shape, spacing)]
-        self.F = [pd.dense_operator(filt, "filter", n, 1.0, True) for n in shape]
-        self.G = [pd.dense_operator("gaussian", "filter", n, 1.0, True) for n in shape]
+        self.periodic = tuple(bool(p) for p in periodic)
+        self.bcs = tuple((0, 0) if p else tuple(b) for p, b in zip(self.periodic, bc))
+        self.kinds = {"d1": derivative, "d2": derivative, "fil": filt, "gfil": "gaussian"}
+        self._cache = {}
         self.tsim = 0.0
         self.step = 0
+
+    def _mat(self, which, a, sign):
+        """Dense operator ``which`` along axis a with the axis's bc pair
+        (negated when sign = -1: a quantity odd at the walls)."""
+        b1, bn = self.bcs[a]
+        key = (which, a, sign * b1, sign * bn)
+        if key not in self._cache:
+            kind, op = self.kinds[which], {"d1": "d1", "d2": "d2", "fil": "filter", "gfil": "filter"}[which]
+            hh = self.h[a] if which in ("d1", "d2") else 1.0
+            self._cache[key] = pd.dense_operator(kind, op, self.shape[a], hh, self.periodic[a], sign * b1, sign * bn)
+        return self._cache[key]
 
     # dense operator along axis a of a 3-D array
     @staticmethod
     def _along(M, f, a):
         return np.moveaxis(np.tensordot(M, f, axes=([1], [a])), 0, a)
 
-    def d1(self, f, a):
-        return self._along(self.D1[a], f, a)
+    def d1(self, f, a, sign=1):
+        return self._along(self._mat("d1", a, sign), f, a)
 
     def d4(self, f, a):
-        return self._along(self.D2[a], self._along(self.D2[a], f, a), a)
+        D2 = self._mat("d2", a, 1)
+        return self._along(D2, self._along(D2, f, a), a)
 
-    def grad(self, f):
-        return [self.d1(f, a) for a in range(3)]
+    def grad(self, f, flip=None):
+        """cgrid.F90:446-478; the bc of axis ``flip`` negated."""
+        return [self.d1(f, a, -1 if a == flip else 1) for a in range(3)]
 
-    def filt(self, f, mats, numtimes=1):
+    def filt(self, f, which, numtimes=1, flip=None):
         """cgrid.F90:1259-1336: y, then x, then z."""
         for a in (1, 0, 2):
+            M = self._mat(which, a, -1 if a == flip else 1)
             for _ in range(numtimes):
-                f = self._along(mats[a], f, a)
+                f = self._along(M, f, a)
         return f
 
     # EOS (IdealGasEOS.F90:45-79)
@@ -108,7 +126,7 @@ class Grid:
         mustar = self.d4(func, 0) * hx ** 6
         mustar = mustar + self.d4(func, 2) * hz ** 6
         mustar = mustar + self.d4(func, 1) * hy ** 6
-        mustar = self.filt(self.Cmu * rho * np.abs(mustar), self.G, 2)
+        mustar = self.filt(self.Cmu * rho * np.abs(mustar), "gfil", 2)
         func = dudx + dvdy + dwdz
         g1, g2, g3 = [g * g for g in self.grad(rho)]
         den = g1 + g2 + g3 + 1.0e-32
@@ -118,7 +136,7 @@ class Grid:
         curl2 = (dwdy - dvdz) ** 2 + (dudz - dwdx) ** 2 + (dvdx - dudy) ** 2
         fsw = func * func / (func * func + curl2 + 1.0e-32)
         fsw = np.where(func >= 0.0, 0.0, fsw)
-        bulk = self.filt(self.Cbeta * rho * fsw * np.abs(bulk), self.G, 2)
+        bulk = self.filt(self.Cbeta * rho * fsw * np.abs(bulk), "gfil", 2)
         e = s["e"]
         g1, g2, g3 = [g * g for g in self.grad(e)]
         den = g1 + g2 + g3 + 1.0e-32
@@ -126,12 +144,12 @@ class Grid:
         kap = kap + self.d4(e, 2) * hz ** 4 * (hz * g3 / den)
         kap = kap + self.d4(e, 1) * hy ** 4 * (hy * g2 / den)
         sos = np.sqrt(self.gam * s["p"] / rho)
-        kap = self.filt(self.Ckap * rho * sos * np.abs(kap) / s["T"], self.G, 2)
+        kap = self.filt(self.Ckap * rho * sos * np.abs(kap) / s["T"], "gfil", 2)
         return mustar, bulk, kap
 
     def rhs(self, W, s, lad=True):
         """getRHS, cgrid.F90:822-1056 (get_tau :1338-1390, get_q :1392-1444)."""
-        du = [self.grad(s[c]) for c in ("u", "v", "w")]
+        du = [self.grad(s[c], flip=a) for a, c in enumerate(("u", "v", "w"))]
         (dudx, dudy, dudz), (dvdx, dvdy, dvdz), (dwdx, dwdy, dwdz) = du
         mu, bulk, kap = self.transport(s)
         if lad:
@@ -153,11 +171,11 @@ class Grid:
         fz = [rw, rw * u - txz, rw * v - tyz, rw * w + p - tzz, (TE + p - tzz) * w - u * txz - v * tyz + qz]
         R = np.zeros(self.shape + (5,))
         for k in range(5):
-            R[..., k] = R[..., k] - self.d1(fx[k], 0)
+            R[..., k] = R[..., k] - self.d1(fx[k], 0, 1 if k == 1 else -1)
         for k in range(5):
-            R[..., k] = R[..., k] - self.d1(fy[k], 1)
+            R[..., k] = R[..., k] - self.d1(fy[k], 1, 1 if k == 2 else -1)
         for k in range(5):
-            R[..., k] = R[..., k] - self.d1(fz[k], 2)
+            R[..., k] = R[..., k] - self.d1(fz[k], 2, 1 if k == 3 else -1)
         return R
 
     def advance(self, s, dt, lad=True):
@@ -174,7 +192,7 @@ class Grid:
             W = W + RK45_B[isub] * Qtmp
             self.tsim += RK45_B[isub] * Qtmpt
             for k in range(5):
-                W[..., k] = self.filt(W[..., k], self.F, 1)
+                W[..., k] = self.filt(W[..., k], "fil", 1, (None, 0, 1, 2, None)[k])
             s = self.from_p(self.primitive(W))
         self.step += 1
         return s

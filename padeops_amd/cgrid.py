@@ -2,10 +2,11 @@
 src/compressible/cgrid.F90 (advance_RK45 :663-717, getRHS :822-1056, LAD
 :1058-1257, filter :1259-1336, get_tau / get_q :1338-1457) for one ideal-gas
 species with the power-law viscosity model (IdealGasEOS.F90,
-PowerLawViscosity.F90, MixtureEOS.F90 ns = 1) on a periodic grid.
+PowerLawViscosity.F90, MixtureEOS.F90 ns = 1), periodic or bounded per
+axis.
 
 Every stage runs in libpadeops_hip: the derivatives and filters (cd10 first
-and second derivatives, cf90 and the periodic Gaussian filter), and the
+and second derivatives, cf90 and the Gaussian filter), and the
 pointwise algebra between them as fused streaming kernels (pade_cg_*: the
 conserved / primitive conversions, the LAD sensors and properties, stress,
 heat flux and all fifteen fluxes in one pass, the RK stage).  Sums of
@@ -55,7 +56,7 @@ def _s():
 
 
 class cgrid:
-    """One-species, periodic compressible grid.  ``init`` takes the sizes,
+    """One-species compressible grid, periodic or bounded per axis.  ``init`` takes the sizes,
     spacings and the &CINPUT parameters (cgrid.F90:150-199: gam, Rgas, Cmu,
     Cbeta, Ckap, inviscid; the power-law viscosity mu_ref, T_ref, n, Pr,
     PowerLawViscosity.F90:8-15).  Fields are (nx, ny, nz) FP64 tensors in
@@ -63,7 +64,14 @@ class cgrid:
 
     def __init__(self, nx, ny, nz, dx, dy, dz, derivative="cd10", filt="cf90", gam=1.4, Rgas=1.0,
                  Cmu=0.002, Cbeta=1.75, Ckap=0.01, inviscid=True, mu_ref=1.0, T_ref=1.0, visc_n=0.75,
-                 Pr=0.75, device="cuda:0"):
+                 Pr=0.75, periodic=(True, True, True), bc=((0, 0), (0, 0), (0, 0)), hook_bc=None,
+                 device="cuda:0"):
+        """``periodic`` per axis and the (bc1, bcn) pair per axis of a bounded
+        grid (&CINPUT x_bc1 ... z_bcn, cgrid.F90:298-306: 0 one-sided, 1
+        symmetric, -1 antisymmetric for the symmetric quantities; the
+        reference flips the sign for the velocity component along the axis).
+        ``hook_bc(grid)``, if given, runs after each substep's primitive
+        update, before e and T are taken from p (hook_bc, cgrid.F90:710)."""
         self.nx, self.ny, self.nz = nx, ny, nz
         self.dx, self.dy, self.dz = dx, dy, dz
         self.gam, self.Rgas = gam, Rgas
@@ -71,13 +79,17 @@ class cgrid:
         self.inviscid = inviscid
         self.mu_ref, self.T_ref, self.visc_n, self.Pr = mu_ref, T_ref, visc_n, Pr
         self.device = device
+        self.periodic = tuple(bool(p) for p in periodic)
+        px, py, pz = self.periodic
         self.der = derivatives()
-        self.der.init(nx, ny, nz, dx, dy, dz, True, True, True, derivative, derivative, derivative)
+        self.der.init(nx, ny, nz, dx, dy, dz, px, py, pz, derivative, derivative, derivative)
         self.fil = filters()
-        self.fil.init(nx, ny, nz, True, True, True, filt, filt, filt)
+        self.fil.init(nx, ny, nz, px, py, pz, filt, filt, filt)
         self.gfil = filters()                                   # cgrid.F90:338-340
-        self.gfil.init(nx, ny, nz, True, True, True, "gaussian", "gaussian", "gaussian")
-        self.x_bc = self.y_bc = self.z_bc = (0, 0)
+        self.gfil.init(nx, ny, nz, px, py, pz, "gaussian", "gaussian", "gaussian")
+        self.bcs = tuple((0, 0) if p else (int(b[0]), int(b[1])) for p, b in zip(self.periodic, bc))
+        self.x_bc, self.y_bc, self.z_bc = self.bcs
+        self.hook_bc = hook_bc
         e = self._empty
         self.rho, self.u, self.v, self.w = e(), e(), e(), e()
         self.p, self.T, self.e = e(), e(), e()
@@ -144,22 +156,26 @@ class cgrid:
         torch.mul(self.mu, Cp / self.Pr, out=self.kap)
 
     # ------------------------------------------------------ operators
-    def _op(self, which, axis, f, out, alpha=None, beta=None):
+    def _op(self, which, axis, f, out, alpha=None, beta=None, sign=1):
         """One derivative along ``axis`` (0, 1, 2) of a (nx, ny, nz) field,
-        optionally accumulated: out = alpha * d(f) + beta * out."""
+        optionally accumulated: out = alpha * d(f) + beta * out; ``sign`` -1
+        takes the axis's bc pair negated (an odd quantity at the walls)."""
         name = "xyz"[axis]
         if which == PADE_D2 and {"x": self.der.getMethodx, "y": self.der.getMethody,
                                  "z": self.der.getMethodz}[name]() == "cd06":
             raise PadeError(21, "CD06 is incomplete right now")      # derivatives.F90:524-525
         sz = (self.nx, self.ny, self.nz)
         na, nb = [s for i, s in enumerate(sz) if i != axis]
-        bc = (self.x_bc, self.y_bc, self.z_bc)[axis]
-        self.der.operator(name)._apply(which, axis + 1, f, out, na, nb, bc[0], bc[1], alpha=alpha, beta=beta)
+        b1, bn = self.bcs[axis]
+        self.der.operator(name)._apply(which, axis + 1, f, out, na, nb, sign * b1, sign * bn,
+                                       alpha=alpha, beta=beta)
 
-    def gradient(self, f, g):
-        """cgrid.F90:446-478 into the three planes of g (y, x, z order)."""
+    def gradient(self, f, g, flip=None):
+        """cgrid.F90:446-478 into the three planes of g (y, x, z order);
+        ``flip`` names the axis whose bc is negated (the gradient of a
+        velocity component along its own axis, cgrid.F90:838-840)."""
         for a in (1, 0, 2):
-            self._op(PADE_D1, a, f, g[:, :, :, a])
+            self._op(PADE_D1, a, f, g[:, :, :, a], sign=-1 if a == flip else 1)
 
     def _d4(self, f, axis, t, out, scale=None):
         """Fourth derivative along one axis as the second derivative applied
@@ -170,15 +186,17 @@ class cgrid:
         else:
             self._op(PADE_D2, axis, t, out, alpha=scale, beta=1.0)
 
-    def filter(self, arr, fil, numtimes=1):
-        """cgrid.F90:1259-1336: y, then x, then z, each ``numtimes`` times;
-        the result replaces ``arr`` (buffers alternate, no copies)."""
-        seq = [f1 for f1 in (fil.filtery, fil.filterx, fil.filterz) for _ in range(numtimes)]
+    def filter(self, arr, fil, numtimes=1, flip=None):
+        """cgrid.F90:1259-1336: y, then x, then z, each ``numtimes`` times,
+        with each axis's bc pair (negated on axis ``flip``); the result
+        replaces ``arr`` (buffers alternate, no copies)."""
+        seq = [(a, f1) for a, f1 in ((1, fil.filtery), (0, fil.filterx), (2, fil.filterz)) for _ in range(numtimes)]
         bufs = (self._t1, self._t2)
         src = arr
-        for k, f1 in enumerate(seq):
+        for k, (a, f1) in enumerate(seq):
             dst = arr if k == len(seq) - 1 else bufs[k % 2]
-            f1(src, dst)
+            s = -1 if a == flip else 1
+            f1(src, dst, s * self.bcs[a][0], s * self.bcs[a][1])
             src = dst
 
     # ------------------------------------------------------------- LAD
@@ -224,9 +242,9 @@ class cgrid:
         (:1338-1390) and get_q (:1392-1444).  ``rhs`` is (nx, ny, nz, 5)."""
         lib, N, st = _lib.load(), self.N, _s()
         du = self._planes(9)
-        self.gradient(self.u, du[:, :, :, 0:3])
-        self.gradient(self.v, du[:, :, :, 3:6])
-        self.gradient(self.w, du[:, :, :, 6:9])
+        self.gradient(self.u, du[:, :, :, 0:3], flip=0)
+        self.gradient(self.v, du[:, :, :, 3:6], flip=1)
+        self.gradient(self.w, du[:, :, :, 6:9], flip=2)
         self.get_transport_properties()
         self.getLAD(du)
         gT = self._planes(3)
@@ -236,12 +254,15 @@ class cgrid:
                                       _p(self.mu), _p(self.bulk), _p(self.kap), _p(gT), _p(fl), st))
         del du, gT
         # rhs = -d(fx)/dx (the x fluxes come negated), then rhs -= d(fy)/dy,
-        # d(fz)/dz (getRHS_x, _y, _z), accumulated inside the derivative kernel
+        # d(fz)/dz (getRHS_x, _y, _z), accumulated inside the derivative kernel;
+        # every flux is odd at the walls except the momentum flux along the
+        # derivative's own axis
         for k in range(5):
-            self._op(PADE_D1, 0, fl[:, :, :, k], rhs[:, :, :, k])
+            self._op(PADE_D1, 0, fl[:, :, :, k], rhs[:, :, :, k], sign=1 if k == 1 else -1)
         for a in (1, 2):
             for k in range(5):
-                self._op(PADE_D1, a, fl[:, :, :, 5 * a + k], rhs[:, :, :, k], alpha=-1.0, beta=1.0)
+                self._op(PADE_D1, a, fl[:, :, :, 5 * a + k], rhs[:, :, :, k], alpha=-1.0, beta=1.0,
+                         sign=1 if k == 1 + a else -1)
 
     # ------------------------------------------------------------ step
     def advance_RK45(self, dt=None):
@@ -266,9 +287,13 @@ class cgrid:
                                             _s()))
             Qtmpt = self.dt + RK45_A[isub] * Qtmpt
             self.tsim += RK45_B[isub] * Qtmpt
-            for k in range(5):
-                self.filter(self.W[:, :, :, k], self.fil, 1)
+            # the momentum component along an axis is odd at its walls (:701-706)
+            for k, flip in zip(range(5), (None, 0, 1, 2, None)):
+                self.filter(self.W[:, :, :, k], self.fil, 1, flip=flip)
             self.get_primitive()
+            if self.hook_bc is not None:
+                self.hook_bc(self)
+                self.post_bc()
         self.step += 1
 
     def get_dt(self, CFL):

@@ -1,9 +1,10 @@
 """The compressible Runge-Kutta step (padeops_amd.cgrid, mirroring
 src/compressible/cgrid.F90) on the GPU against the CPU oracle
 (oracle/cgrid_step.py): the right-hand side with the artificial properties
-and viscosity on a non-cubic grid, and a full five-substep step of the
-Taylor-Green problem, both to 1e-12; the periodic Gaussian filter through
-the C ABI; conservation of one step at 128^3."""
+and viscosity on a non-cubic grid, full five-substep steps of a generic state
+on periodic and bounded grids (one-sided and mirror bc pairs), both to 1e-12,
+and of the Taylor-Green problem; the Gaussian filter through the C ABI;
+conservation of one step at 128^3."""
 import ctypes
 import math
 
@@ -96,6 +97,29 @@ def test_full_step_matches_oracle():
     torch.cuda.synchronize()
     _compare_step(g, s1, 1e-12)
     assert abs(g.tsim - og.tsim) < 1e-15
+    g.destroy()
+
+
+@pytest.mark.parametrize("periodic,bc", [((False, True, True), ((0, 0), (0, 0), (0, 0))),
+                                         ((True, False, False), ((0, 0), (1, -1), (-1, 1)))])
+def test_bounded_step_matches_oracle(periodic, bc):
+    # bounded axes with one-sided (0) or mirror (+-1) closures; the velocity
+    # along an axis, its momentum and the fluxes take the pair negated
+    shape = (20, 16, 18)
+    s0, h = _smooth(shape, 21)
+    h = tuple(1.0 / (n - 1) if not p else hh for n, p, hh in zip(shape, periodic, h))
+    kw = dict(gam=1.4, Rgas=1.0, inviscid=False, mu_ref=0.01, T_ref=1.0, visc_n=0.75, Pr=0.7,
+              periodic=periodic, bc=bc)
+    og = cg.Grid(shape, h, **kw)
+    s1 = og.advance(s0, 1e-4)
+    g = gcg.cgrid(*shape, *h, **kw)
+    for c in ("rho", "u", "v", "w", "p"):
+        getattr(g, c).copy_(_to_gpu(s0[c]))
+    g.post_bc()
+    g.get_conserved()
+    g.advance_RK45(1e-4)
+    torch.cuda.synchronize()
+    _compare_step(g, s1, 1e-12)
     g.destroy()
 
 

@@ -11,7 +11,9 @@
   * the periodic Gaussian filter: its transfer function
     agf + 2 sum_m c_m cos(m k), and unit gain at k = 0;
   * the full step with LAD and the filter conserves mass, momentum and
-    energy and keeps a uniform state.
+    energy and keeps a uniform state;
+  * bounded axes: the step with the symmetric bc pair equals the periodic
+    step of the mirror-extended state on its half.
 """
 import numpy as np
 import pytest
@@ -145,3 +147,32 @@ def test_step_conserves_and_keeps_uniform_state():
     s1 = g.advance(u, 1e-2)
     for c in ("rho", "u", "v", "w", "p"):
         assert np.abs(s1[c] - u[c]).max() < 1e-12 * np.abs(u[c]).max(), c
+
+
+def test_bounded_mirror_step_is_half_of_periodic_step():
+    """Bounded axes with the symmetric pair (1, 1) on a state that is even
+    about x = 0, pi and y = 0, pi (u odd in x, v odd in y): one full step with
+    LAD and viscosity equals the periodic step on the doubled grid restricted
+    to the half (nodes 0..n/2).  A wrong flux, gradient or filter sign flip
+    (cgrid.F90:446-478, :663-717, getRHS_x/y/z) breaks the symmetry."""
+    rng = np.random.default_rng(9)
+    full, half = (32, 24, 8), (17, 13, 8)
+    h = tuple(2 * np.pi / n for n in full)
+    X, Y, Z = np.meshgrid(*[np.arange(n) * hh for n, hh in zip(full, h)], indexing="ij")
+
+    def modes(a0, amp, fx, fy):
+        f = np.full(full, a0)
+        for _ in range(3):
+            kx, ky, kz = rng.integers(1, 3, size=3)
+            f = f + amp * rng.standard_normal() * fx(kx * X) * fy(ky * Y) * np.cos(kz * Z + rng.uniform(0, 6.28))
+        return f
+    s = dict(rho=modes(1.0, 0.05, np.cos, np.cos), u=modes(0.0, 0.2, np.sin, np.cos),
+             v=modes(0.0, 0.2, np.cos, np.sin), w=modes(0.1, 0.2, np.cos, np.cos),
+             p=modes(2.0, 0.1, np.cos, np.cos))
+    kw = dict(gam=1.4, Rgas=1.0, inviscid=False, mu_ref=0.02, T_ref=1.0, visc_n=0.75, Pr=0.7)
+    sp = cg.Grid(full, h, **kw).advance(s, 2e-3)
+    sh = {c: s[c][:17, :13].copy() for c in s}
+    sb = cg.Grid(half, h, periodic=(False, False, True), bc=((1, 1), (1, 1), (0, 0)), **kw).advance(sh, 2e-3)
+    for c in ("rho", "u", "v", "w", "p", "T"):
+        err = np.abs(sb[c] - sp[c][:17, :13]).max() / np.abs(sp[c]).max()
+        assert err < 1e-12, (c, err)

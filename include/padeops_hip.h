@@ -211,7 +211,9 @@ int pade_plan_export(int kind, int oper, int n, double dx, int periodic, int bc1
  * between them are the operator calls above.  Return 0 or 400 / 1000 + hip
  * error. */
 int pade_cg_conserved(long long n, const double *rho, const double *u, const double *v, const double *w,
-                      const double *e, double *W, void *stream);                  /* cgrid.F90:799-811 */
+                      const double *e, double *W, int *nan_flag,
+                      void *stream);   /* cgrid.F90:799-811; nan_flag (device int, may be NULL): set to 1
+                                          when any W value is NaN, the test of :684-692 */
 int pade_cg_primitive(long long n, const double *W, double gam, double Rgas, double *rho, double *u, double *v,
                       double *w, double *e, double *p, double *T, void *stream);  /* :764-797, :813-820 */
 int pade_cg_strain(long long n, const double *du, double *func, double *dil, void *stream);  /* :1085-1087, :1117 */

@@ -74,7 +74,7 @@ def load():
     lib.pade_plan_export.argtypes = [c_int, c_int, c_int, c_double, c_int, c_int, c_int, c_int, c_int, dp,
                                      c_ll, ip]
     v, d = c_void_p, c_double
-    lib.pade_cg_conserved.argtypes = [c_ll, v, v, v, v, v, v, v]
+    lib.pade_cg_conserved.argtypes = [c_ll, v, v, v, v, v, v, v, v]
     lib.pade_cg_primitive.argtypes = [c_ll, v, d, d, v, v, v, v, v, v, v, v]
     lib.pade_cg_strain.argtypes = [c_ll, v, v, v, v]
     lib.pade_cg_lad_mu.argtypes = [c_ll, d, v, v, v]

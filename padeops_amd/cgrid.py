@@ -96,6 +96,7 @@ class cgrid:
         self.mu, self.bulk, self.kap = e(), e(), e()
         self.W = fortran_empty((nx, ny, nz, 5), device=device)
         self._t1, self._t2 = e(), e()        # work fields of the operator chains
+        self._nan = torch.zeros(1, dtype=torch.int32, device=device)   # set by get_conserved on a NaN
         self.tsim = 0.0
         self.step = 0
         self.dt = 0.0
@@ -138,10 +139,16 @@ class cgrid:
         _lib.check(_lib.load().pade_cg_primitive(self.N, _p(self.W), self.gam, self.Rgas, _p(self.rho), _p(self.u),
                                                  _p(self.v), _p(self.w), _p(self.e), _p(self.p), _p(self.T), _s()))
 
-    def get_conserved(self):
-        """cgrid.F90:799-811, one kernel."""
+    def get_conserved(self, nan_check=False):
+        """cgrid.F90:799-811, one kernel; with ``nan_check`` the same pass
+        tests W for NaN (the test of advance_RK45, :684-692) and the return
+        value says whether it found one."""
+        if nan_check:
+            self._nan.zero_()
         _lib.check(_lib.load().pade_cg_conserved(self.N, _p(self.rho), _p(self.u), _p(self.v), _p(self.w),
-                                                 _p(self.e), _p(self.W), _s()))
+                                                 _p(self.e), _p(self.W), _p(self._nan) if nan_check else None,
+                                                 _s()))
+        return bool(self._nan.item()) if nan_check else False
 
     def get_transport_properties(self):
         """MixtureEOS.F90:175-229 (ns = 1) with PowerLawViscosity.F90:38-75."""
@@ -279,8 +286,7 @@ class cgrid:
         Qtmp.zero_()
         Qtmpt = 0.0
         for isub in range(5):
-            self.get_conserved()
-            if not bool(torch.isfinite(self.W).all()):
+            if self.get_conserved(nan_check=True):
                 raise FloatingPointError(f"NaN encountered in solution at substep {isub + 1} of step {self.step + 1}")
             self.getRHS(rhs)
             _lib.check(lib.pade_cg_rk_stage(N, self.dt, RK45_A[isub], RK45_B[isub], _p(rhs), _p(Qtmp), _p(self.W),

@@ -30,17 +30,24 @@ inline unsigned grid_for(long long n)
 __global__ __launch_bounds__(kBlock) void k_conserved(long long n, const double *__restrict__ rho,
                                                       const double *__restrict__ u, const double *__restrict__ v,
                                                       const double *__restrict__ w, const double *__restrict__ e,
-                                                      double *__restrict__ W)
+                                                      double *__restrict__ W, int *__restrict__ nan_flag)
 {
+    bool nan = false;
     GRID_STRIDE(i, n)
     {
         const double r = rho[i], uu = u[i], vv = v[i], ww = w[i];
+        const double w1 = r * uu, w2 = r * vv, w3 = r * ww;
+        const double w4 = r * (e[i] + 0.5 * (uu * uu + vv * vv + ww * ww));
         W[i] = r;
-        W[n + i] = r * uu;
-        W[2 * n + i] = r * vv;
-        W[3 * n + i] = r * ww;
-        W[4 * n + i] = r * (e[i] + 0.5 * (uu * uu + vv * vv + ww * ww));
+        W[n + i] = w1;
+        W[2 * n + i] = w2;
+        W[3 * n + i] = w3;
+        W[4 * n + i] = w4;
+        nan |= (r != r) | (w1 != w1) | (w2 != w2) | (w3 != w3) | (w4 != w4);
     }
+    // the NaN test of advance_RK45 (cgrid.F90:684-692) fused into the pass
+    // (a failure path: only lanes that saw a NaN touch the flag)
+    if (nan_flag && nan) atomicOr(nan_flag, 1);
 }
 
 // get_primitive (cgrid.F90:764-797) followed by post_bc (:813-820):
@@ -212,10 +219,10 @@ using namespace pade;
 extern "C" {
 
 int pade_cg_conserved(long long n, const double *rho, const double *u, const double *v, const double *w,
-                      const double *e, double *W, void *stream)
+                      const double *e, double *W, int *nan_flag, void *stream)
 {
     if (n < 1 || !rho || !u || !v || !w || !e || !W) return 400;
-    hipLaunchKernelGGL(k_conserved, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, n, rho, u, v, w, e, W);
+    hipLaunchKernelGGL(k_conserved, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, n, rho, u, v, w, e, W, nan_flag);
     return launched();
 }
 

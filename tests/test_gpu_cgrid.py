@@ -112,7 +112,10 @@ def test_bounded_step_matches_oracle(periodic, bc):
               periodic=periodic, bc=bc)
     og = cg.Grid(shape, h, **kw)
     s1 = og.advance(s0, 1e-4)
-    g = gcg.cgrid(*shape, *h, **kw)
+    calls = []
+    # a hook that leaves the state alone: it runs once per substep, after the
+    # primitive update
+    g = gcg.cgrid(*shape, *h, hook_bc=lambda grid: calls.append(grid.tsim), **kw)
     for c in ("rho", "u", "v", "w", "p"):
         getattr(g, c).copy_(_to_gpu(s0[c]))
     g.post_bc()
@@ -120,6 +123,7 @@ def test_bounded_step_matches_oracle(periodic, bc):
     g.advance_RK45(1e-4)
     torch.cuda.synchronize()
     _compare_step(g, s1, 1e-12)
+    assert len(calls) == 5 and abs(calls[-1] - g.tsim) < 1e-18
     g.destroy()
 
 
@@ -192,3 +196,20 @@ def test_gaussian_filter_c_abi(n):
         lib.pade_op_destroy(h)
     # shorter than the 9-point stencil: 21
     assert lib.pade_op_create(_lib.PADE_GAUSSIAN, 8, 0.0, 1, 0, 0, ctypes.byref(h)) == 21
+
+
+def test_nan_in_state_stops_the_step():
+    # the NaN test of advance_RK45 (cgrid.F90:684-692), fused into the
+    # conserved-variable pass: a NaN anywhere in W raises before the RHS
+    shape = (12, 10, 16)
+    s0, h = _smooth(shape, 5)
+    g = gcg.cgrid(*shape, *h, inviscid=False, mu_ref=0.01)
+    for c in ("rho", "u", "v", "w", "p"):
+        getattr(g, c).copy_(_to_gpu(s0[c]))
+    g.post_bc()
+    assert g.get_conserved(nan_check=True) is False
+    g.advance_RK45(1e-4)                     # a clean state steps
+    g.v[7, 3, 5] = float("nan")
+    with pytest.raises(FloatingPointError, match="substep 1 of step 2"):
+        g.advance_RK45(1e-4)
+    g.destroy()

@@ -143,7 +143,7 @@ def test_compressible_kernels_reject_bad_arguments():
     (no GPU call): empty fields or null arrays give 400."""
     lib = _lib.load()
     nul = None
-    assert lib.pade_cg_conserved(0, nul, nul, nul, nul, nul, nul, nul) == 400
+    assert lib.pade_cg_conserved(0, nul, nul, nul, nul, nul, nul, nul, nul) == 400
     assert lib.pade_cg_primitive(8, nul, 1.4, 1.0, nul, nul, nul, nul, nul, nul, nul, nul) == 400
     assert lib.pade_cg_strain(8, nul, nul, nul, nul) == 400
     assert lib.pade_cg_lad_mu(-1, 0.002, nul, nul, nul) == 400

@@ -27,7 +27,7 @@ import torch
 import ctypes
 
 from . import _lib
-from ._lib import PADE_D1, PADE_D2, PadeError
+from ._lib import PADE_D1, PADE_D2, PADE_FILTER, PadeError
 from .facades import derivatives, filters
 from .operators import as_fortran, fortran_empty
 
@@ -193,16 +193,21 @@ class cgrid:
         else:
             self._op(PADE_D2, axis, t, out, alpha=scale, beta=1.0)
 
-    def filter(self, arr, fil, numtimes=1, flip=None):
+    def filter(self, arr, fil, numtimes=1, flip=None, add_to=None):
         """cgrid.F90:1259-1336: y, then x, then z, each ``numtimes`` times,
         with each axis's bc pair (negated on axis ``flip``); the result
-        replaces ``arr`` (buffers alternate, no copies)."""
+        replaces ``arr`` (buffers alternate, no copies), or with ``add_to``
+        the last pass adds it to that field instead (add_to += filtered)."""
         seq = [(a, f1) for a, f1 in ((1, fil.filtery), (0, fil.filterx), (2, fil.filterz)) for _ in range(numtimes)]
         bufs = (self._t1, self._t2)
         src = arr
         for k, (a, f1) in enumerate(seq):
-            dst = arr if k == len(seq) - 1 else bufs[k % 2]
             s = -1 if a == flip else 1
+            if k == len(seq) - 1 and add_to is not None:
+                fil.operator("z")._apply(PADE_FILTER, 3, src, add_to, self.nx, self.ny, s * self.bcs[a][0],
+                                         s * self.bcs[a][1], alpha=1.0, beta=1.0)
+                return
+            dst = arr if k == len(seq) - 1 else bufs[k % 2]
             f1(src, dst, s * self.bcs[a][0], s * self.bcs[a][1])
             src = dst
 
@@ -220,7 +225,7 @@ class cgrid:
         for a, h in ((0, hx), (2, hz), (1, hy)):
             self._d4(func, a, t, mustar, scale=h ** 6)
         _lib.check(lib.pade_cg_lad_mu(N, self.Cmu, _p(self.rho), _p(mustar), st))
-        self.filter(mustar, self.gfil, 2)
+        self.filter(mustar, self.gfil, 2, add_to=self.mu)     # (:1252)
         # artificial bulk viscosity (:1115-1161)
         g, d4 = self._planes(3), self._planes(3)
         self.gradient(self.rho, g)
@@ -229,7 +234,7 @@ class cgrid:
         bulkstar = func          # the sensor is no longer needed
         _lib.check(lib.pade_cg_lad_bulk(N, self.Cbeta, hx, hy, hz, _p(self.rho), _p(g), _p(d4), _p(du), _p(dil),
                                         _p(bulkstar), st))
-        self.filter(bulkstar, self.gfil, 2)
+        self.filter(bulkstar, self.gfil, 2, add_to=self.bulk)  # (:1253)
         # artificial conductivity (:1163-1199)
         self.gradient(self.e, g)
         for a in (0, 2, 1):
@@ -237,11 +242,7 @@ class cgrid:
         kapstar = dil
         _lib.check(lib.pade_cg_lad_kap(N, self.Ckap, self.gam, hx, hy, hz, _p(self.rho), _p(self.p), _p(self.T),
                                        _p(g), _p(d4), _p(kapstar), st))
-        self.filter(kapstar, self.gfil, 2)
-        # (:1252-1254)
-        self.mu += mustar
-        self.bulk += bulkstar
-        self.kap += kapstar
+        self.filter(kapstar, self.gfil, 2, add_to=self.kap)    # (:1254)
 
     # ------------------------------------------------------------- RHS
     def getRHS(self, rhs):
